@@ -1,0 +1,188 @@
+// Internal declarations shared by the HIP sources of libamcl3d_gpu.
+//
+// Device layout (HBM):
+//  * grid: prob as one float per cell, x fastest, index ix + iy*sx + iz*sx*sy
+//    (the reference's cell order, Grid3d.cpp:187), 64-bit when the map has
+//    2^32 cells or more; the squared distance field is a second, optional,
+//    array of the same shape.
+//  * filter: structure of arrays x, y, z, a, w, wp, wr (one float each per
+//    particle, the fields of struct Particle, ParticleFilter.h:35-49).
+//  * cloud: three float arrays x, y, z in cloud order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/amcl3d_gpu.h"
+
+namespace a3d {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+bool hip_ok(hipError_t e, const char* what);
+
+#define A3D_HIP(call)                                      \
+  do {                                                     \
+    if (!::a3d::hip_ok((call), #call)) return A3D_ERR_HIP; \
+  } while (0)
+
+// ------------------------------------------------------------ device views
+struct GridDev {
+  const float* prob;
+  float* dist;  // optional (map build output)
+  uint32_t size[3];
+  uint32_t idx64;
+  uint64_t step_z;  // sx*sy
+  uint64_t cells;
+  double min[3];
+  double max[3];
+  double ext[3];    // max - min, as Grid3d.cpp:151-153
+  float ext_hi[3];  // for a float q: (double)q < ext  <=>  q < ext_hi
+  float pad_;
+  double res;
+  double inv_res;
+};
+
+// One filter and the parameters of the call being made on it.  Arrays are
+// device pointers.
+struct FilterDev {
+  float* x;
+  float* y;
+  float* z;
+  float* a;
+  float* w;
+  float* wp;
+  float* wr;
+  float* scratch;  // scratch_floats(n): tail sums, resample prefix, gather staging
+  uint8_t* inmap;  // n
+  float* mean;     // 4
+  uint32_t n;
+  // update
+  uint32_t m;
+  uint32_t nb;
+  uint32_t rp0;  // roll == pitch == 0
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  const float* ranges;  // nb x {r, ax, ay, az}
+  float roll;
+  float pitch;
+  double alpha;
+  double sigma;
+  // predict / init draws: 4 standard normals per particle
+  const float* z4;
+  double delta[4];  // predict: delta x, y, z, a
+  double dev[4];    // predict: fabs(delta*mod); init: x, y, z, a deviations
+  float pose[4];    // init
+  // resample
+  float u01;
+  uint32_t* idx_out;  // optional: ancestor indices
+};
+
+// A unit of cloud-weight work: particles [p0, p0+count) of one filter.
+struct WorkItem {
+  uint32_t filter;
+  uint32_t p0;
+  uint32_t count;
+  uint32_t pad_;
+};
+
+// Filters a launch works on: one descriptor by value, or a device array.
+struct Filters {
+  const FilterDev* arr;  // device array, or null
+  uint32_t count;        // number of filters
+  uint32_t max_n;        // largest particle count
+  FilterDev one;         // used when arr == null
+};
+
+inline Filters single(const FilterDev& f) {
+  Filters s{};
+  s.arr = nullptr;
+  s.count = 1;
+  s.max_n = f.n;
+  s.one = f;
+  return s;
+}
+
+// Host-side grid metadata, computed exactly as the reference does.
+void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g);
+
+// ------------------------------------------------------------ launchers
+// Cloud weight (Grid3d::computeCloudWeight for every in-map particle) and the
+// range weight; writes wp, wr of in-map particles (out-of-map ones keep their
+// previous wp, wr as in ParticleFilter.cpp:137-141).
+void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
+                   uint32_t* d_counter, int ppw, int persistent, hipStream_t s);
+// Poses given explicitly (a3d_grid_cloud_weight): out[i].
+void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, const float* cz, uint32_t m,
+                         const float* tx, const float* ty, const float* tz, const float* yaw, uint32_t n, float roll,
+                         float pitch, float* out, int ppw, hipStream_t s);
+// Normalisation, blend and mean of ParticleFilter::update (ParticleFilter.cpp:150-195).
+void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s);
+void launch_predict(const Filters& fs, hipStream_t s);
+void launch_init(const Filters& fs, hipStream_t s);
+void launch_resample(const Filters& fs, int mode, hipStream_t s);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+// Device map build (computeGrid, PointCloudTools.cpp:84-149).
+int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_prob, float* d_dist,
+                      double sensor_dev, hipStream_t s);
+
+// Counter-based draws for served sessions (Philox4x32-10 + Box-Muller).
+// Writes 4 normals per particle to each filter's z4 and one uniform per
+// filter to d_u01 (may be null).
+void launch_draws(const Filters& fs, const uint64_t* d_seed, const uint64_t* d_step, float* d_u01, hipStream_t s);
+
+// Cloud-weight kernel tile sizes.
+int clamp_ppw(int ppw);
+int current_ppw();
+size_t scratch_floats(uint32_t n);
+
+// Event pairs around the cloud-weight launches, resolved on demand.
+struct TimingPool {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
+  double acc_ms = 0.0;
+  uint64_t launches = 0;
+  uint64_t evals = 0;
+
+  std::pair<hipEvent_t, hipEvent_t> take() {
+    if (!spare.empty()) {
+      auto e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+    if (hipEventCreate(&e.first) != hipSuccess) return {nullptr, nullptr};
+    if (hipEventCreate(&e.second) != hipSuccess) {
+      (void)hipEventDestroy(e.first);
+      return {nullptr, nullptr};
+    }
+    return e;
+  }
+  void resolve() {
+    for (auto& e : pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) acc_ms += ms;
+      spare.push_back(e);
+    }
+    pending.clear();
+  }
+  void destroy() {
+    for (auto& e : pending) spare.push_back(e);
+    pending.clear();
+    for (auto& e : spare) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    spare.clear();
+  }
+};
+
+}  // namespace a3d

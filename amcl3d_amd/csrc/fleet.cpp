@@ -1,0 +1,384 @@
+// Fleet: many independent particle filters on one resident grid.
+//
+// All filters' particles live in one structure-of-arrays allocation (field k
+// of filter f at state + k*total + offset[f]).  An update stages every
+// filter's cloud and ranges, then runs ONE cloud-weight launch over a work
+// list of (filter, block of particles) items in decreasing cost, taken by
+// persistent wavefronts from an atomic counter, and one tail launch with a
+// workgroup per filter.  Each filter's arithmetic is exactly that of the
+// single-filter path, so results do not depend on the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "a3d_handles.h"
+
+using namespace a3d;
+
+struct a3d_fleet {
+  hipStream_t stream = nullptr;
+  uint32_t F = 0;
+  std::vector<uint32_t> n;
+  std::vector<uint64_t> poff;  // F + 1
+  std::vector<uint64_t> soff;  // scratch offsets (floats)
+  uint64_t total = 0;
+  uint32_t max_n = 0;
+  float* state = nullptr;
+  float* scratch = nullptr;
+  uint8_t* inmap = nullptr;
+  float* mean = nullptr;
+  FilterDev* d_filters = nullptr;
+  std::vector<FilterDev> h_filters;
+  // staging
+  float* stage = nullptr;
+  size_t stage_cap = 0;
+  // staged update
+  float* cloud = nullptr;
+  size_t cloud_cap = 0;
+  WorkItem* d_items = nullptr;
+  size_t items_cap = 0;
+  uint32_t nitems = 0;
+  uint32_t* d_counter = nullptr;
+  bool staged = false;
+  std::vector<uint32_t> st_m, st_nb;
+  std::vector<double> st_alpha, st_sigma, st_roll, st_pitch;
+  std::vector<uint64_t> st_moff, st_boff;
+  TimingPool timing;
+
+  FilterDev base(uint32_t f) const {
+    FilterDev d{};
+    const uint64_t o = poff[f];
+    d.x = state + 0 * total + o;
+    d.y = state + 1 * total + o;
+    d.z = state + 2 * total + o;
+    d.a = state + 3 * total + o;
+    d.w = state + 4 * total + o;
+    d.wp = state + 5 * total + o;
+    d.wr = state + 6 * total + o;
+    d.scratch = scratch + soff[f];
+    d.inmap = inmap + o;
+    d.mean = mean + 4 * (size_t)f;
+    d.n = n[f];
+    return d;
+  }
+  Filters device_filters() const {
+    Filters s{};
+    s.arr = d_filters;
+    s.count = F;
+    s.max_n = max_n;
+    return s;
+  }
+};
+
+static int grow(float*& buf, size_t& cap, size_t floats, hipStream_t s) {
+  if (floats <= cap) return A3D_OK;
+  if (buf) {
+    A3D_HIP(hipStreamSynchronize(s));
+    (void)hipFree(buf);
+  }
+  buf = nullptr;
+  cap = 0;
+  const size_t want = std::max(floats, (size_t)1 << 16);
+  if (hipMalloc(&buf, want * sizeof(float)) != hipSuccess) return fail(A3D_ERR_NOMEM, "fleet: staging allocation");
+  cap = want;
+  return A3D_OK;
+}
+
+static int upload_filters(a3d_fleet* fl) {
+  A3D_HIP(hipMemcpyAsync(fl->d_filters, fl->h_filters.data(), fl->F * sizeof(FilterDev), hipMemcpyHostToDevice,
+                         fl->stream));
+  return A3D_OK;
+}
+
+extern "C" {
+
+int a3d_fleet_create(size_t nfilters, const uint32_t* num_particles, a3d_fleet** out) {
+  if (!out || !num_particles || nfilters == 0 || nfilters > 0xffffffull)
+    return fail(A3D_ERR_ARG, "a3d_fleet_create: bad argument");
+  *out = nullptr;
+  a3d_fleet* fl = new a3d_fleet();
+  fl->F = (uint32_t)nfilters;
+  fl->n.assign(num_particles, num_particles + nfilters);
+  fl->poff.assign(nfilters + 1, 0);
+  fl->soff.assign(nfilters + 1, 0);
+  for (size_t f = 0; f < nfilters; ++f) {
+    if (fl->n[f] == 0) {
+      delete fl;
+      return fail(A3D_ERR_ARG, "a3d_fleet_create: every filter needs at least one particle");
+    }
+    fl->poff[f + 1] = fl->poff[f] + fl->n[f];
+    fl->soff[f + 1] = fl->soff[f] + ((scratch_floats(fl->n[f]) + 3) & ~(size_t)3);  // keep doubles aligned
+    fl->max_n = std::max(fl->max_n, fl->n[f]);
+  }
+  fl->total = fl->poff[nfilters];
+  if (fl->total > 0xffffffffull) {
+    delete fl;
+    return fail(A3D_ERR_ARG, "a3d_fleet_create: more than 2^32 particles");
+  }
+  bool ok = hipStreamCreateWithFlags(&fl->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&fl->state, 7 * fl->total * 4) == hipSuccess &&
+            hipMalloc(&fl->scratch, fl->soff[nfilters] * 4) == hipSuccess &&
+            hipMalloc(&fl->inmap, fl->total) == hipSuccess && hipMalloc(&fl->mean, 16 * nfilters) == hipSuccess &&
+            hipMalloc(&fl->d_filters, sizeof(FilterDev) * nfilters) == hipSuccess &&
+            hipMalloc(&fl->d_counter, 64) == hipSuccess;
+  if (ok)
+    ok = hipMemsetAsync(fl->state, 0, 7 * fl->total * 4, fl->stream) == hipSuccess &&
+         hipMemsetAsync(fl->mean, 0, 16 * nfilters, fl->stream) == hipSuccess &&
+         hipStreamSynchronize(fl->stream) == hipSuccess;
+  if (!ok) {
+    a3d_fleet_destroy(fl);
+    return fail(A3D_ERR_NOMEM, "a3d_fleet_create: device allocation failed");
+  }
+  fl->h_filters.resize(nfilters);
+  *out = fl;
+  return A3D_OK;
+}
+
+int a3d_fleet_destroy(a3d_fleet* fl) {
+  if (!fl) return A3D_OK;
+  if (fl->stream) (void)hipStreamSynchronize(fl->stream);
+  fl->timing.destroy();
+  (void)hipFree(fl->state);
+  (void)hipFree(fl->scratch);
+  (void)hipFree(fl->inmap);
+  (void)hipFree(fl->mean);
+  (void)hipFree(fl->d_filters);
+  (void)hipFree(fl->d_counter);
+  (void)hipFree(fl->stage);
+  (void)hipFree(fl->cloud);
+  (void)hipFree(fl->d_items);
+  if (fl->stream) (void)hipStreamDestroy(fl->stream);
+  delete fl;
+  return A3D_OK;
+}
+
+int a3d_fleet_set_particles(a3d_fleet* fl, const float* p7) {
+  if (!fl || !p7) return fail(A3D_ERR_ARG, "a3d_fleet_set_particles: null argument");
+  std::vector<float> soa(7 * fl->total);
+  for (uint64_t i = 0; i < fl->total; ++i)
+    for (int k = 0; k < 7; ++k) soa[k * fl->total + i] = p7[7 * i + k];
+  A3D_HIP(hipMemcpyAsync(fl->state, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, fl->stream));
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_get_particles(const a3d_fleet* fl, float* p7) {
+  if (!fl || !p7) return fail(A3D_ERR_ARG, "a3d_fleet_get_particles: null argument");
+  std::vector<float> soa(7 * fl->total);
+  A3D_HIP(hipMemcpyAsync(soa.data(), fl->state, soa.size() * 4, hipMemcpyDeviceToHost, fl->stream));
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  for (uint64_t i = 0; i < fl->total; ++i)
+    for (int k = 0; k < 7; ++k) p7[7 * i + k] = soa[k * fl->total + i];
+  return A3D_OK;
+}
+
+int a3d_fleet_init(a3d_fleet* fl, const float* poses, const float* devs, const float* z4) {
+  if (!fl || !poses || !devs || !z4) return fail(A3D_ERR_ARG, "a3d_fleet_init: null argument");
+  int rc = grow(fl->stage, fl->stage_cap, 4 * fl->total, fl->stream);
+  if (rc) return rc;
+  A3D_HIP(hipMemcpyAsync(fl->stage, z4, 4 * fl->total * 4, hipMemcpyHostToDevice, fl->stream));
+  for (uint32_t f = 0; f < fl->F; ++f) {
+    FilterDev d = fl->base(f);
+    const float* dv = devs + 4 * (size_t)f;
+    const float dev = std::max(std::max(dv[0], dv[1]), dv[2]);  // ParticleFilter.cpp:54-56
+    d.dev[0] = (float)(1. / (dev * std::sqrt(2 * M_PI)));
+    d.dev[1] = (float)(1. / (2 * dev * dev));
+    for (int k = 0; k < 4; ++k) {
+      d.delta[k] = dv[k];
+      d.pose[k] = poses[4 * (size_t)f + k];
+    }
+    d.z4 = fl->stage + 4 * fl->poff[f];
+    fl->h_filters[f] = d;
+  }
+  rc = upload_filters(fl);
+  if (rc) return rc;
+  launch_init(fl->device_filters(), fl->stream);
+  A3D_HIP(hipGetLastError());
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_predict(a3d_fleet* fl, const double* mods, const double* deltas, const float* z4) {
+  if (!fl || !mods || !deltas || !z4) return fail(A3D_ERR_ARG, "a3d_fleet_predict: null argument");
+  int rc = grow(fl->stage, fl->stage_cap, 4 * fl->total, fl->stream);
+  if (rc) return rc;
+  A3D_HIP(hipMemcpyAsync(fl->stage, z4, 4 * fl->total * 4, hipMemcpyHostToDevice, fl->stream));
+  for (uint32_t f = 0; f < fl->F; ++f) {
+    FilterDev d = fl->base(f);
+    for (int k = 0; k < 4; ++k) {
+      d.delta[k] = deltas[4 * (size_t)f + k];
+      d.dev[k] = std::fabs(deltas[4 * (size_t)f + k] * mods[4 * (size_t)f + k]);
+    }
+    d.z4 = fl->stage + 4 * fl->poff[f];
+    fl->h_filters[f] = d;
+  }
+  rc = upload_filters(fl);
+  if (rc) return rc;
+  launch_predict(fl->device_filters(), fl->stream);
+  A3D_HIP(hipGetLastError());
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const float* z, const uint32_t* m,
+                           const float* ranges, const uint32_t* nb, const double* alpha, const double* sigma,
+                           const double* roll, const double* pitch) {
+  if (!fl || !m || !nb || !alpha || !sigma || !roll || !pitch)
+    return fail(A3D_ERR_ARG, "a3d_fleet_stage_update: null argument");
+  std::vector<uint64_t> moff(fl->F + 1, 0), boff(fl->F + 1, 0);
+  for (uint32_t f = 0; f < fl->F; ++f) {
+    moff[f + 1] = moff[f] + m[f];
+    boff[f + 1] = boff[f] + nb[f];
+  }
+  const uint64_t M = moff[fl->F], B = boff[fl->F];
+  if ((M && (!x || !y || !z)) || (B && !ranges)) return fail(A3D_ERR_ARG, "a3d_fleet_stage_update: null data");
+  int rc = grow(fl->cloud, fl->cloud_cap, 3 * M + 4 * B + 16, fl->stream);
+  if (rc) return rc;
+  float* cx = fl->cloud;
+  float* cy = cx + M;
+  float* cz = cy + M;
+  float* rg = cz + M;
+  if (M) {
+    A3D_HIP(hipMemcpyAsync(cx, x, M * 4, hipMemcpyHostToDevice, fl->stream));
+    A3D_HIP(hipMemcpyAsync(cy, y, M * 4, hipMemcpyHostToDevice, fl->stream));
+    A3D_HIP(hipMemcpyAsync(cz, z, M * 4, hipMemcpyHostToDevice, fl->stream));
+  }
+  if (B) A3D_HIP(hipMemcpyAsync(rg, ranges, B * 16, hipMemcpyHostToDevice, fl->stream));
+  for (uint32_t f = 0; f < fl->F; ++f) {
+    FilterDev d = fl->base(f);
+    d.cx = cx + moff[f];
+    d.cy = cy + moff[f];
+    d.cz = cz + moff[f];
+    d.m = m[f];
+    d.ranges = nb[f] ? rg + 4 * boff[f] : nullptr;
+    d.nb = nb[f];
+    d.alpha = alpha[f];
+    d.sigma = sigma[f];
+    d.roll = (float)roll[f];
+    d.pitch = (float)pitch[f];
+    d.rp0 = (d.roll == 0.0f && d.pitch == 0.0f) ? 1u : 0u;
+    fl->h_filters[f] = d;
+  }
+  rc = upload_filters(fl);
+  if (rc) return rc;
+  // Work list: blocks of particles, filters in decreasing cost n*m.
+  const int ppw = clamp_ppw(current_ppw());
+  std::vector<uint32_t> order(fl->F);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)fl->n[a] * m[a] > (uint64_t)fl->n[b] * m[b];
+  });
+  std::vector<WorkItem> items;
+  items.reserve(fl->total / ppw + fl->F);
+  for (uint32_t f : order)
+    for (uint32_t p0 = 0; p0 < fl->n[f]; p0 += ppw) items.push_back({f, p0, std::min<uint32_t>(ppw, fl->n[f] - p0), 0});
+  if (items.size() > fl->items_cap) {
+    A3D_HIP(hipStreamSynchronize(fl->stream));
+    (void)hipFree(fl->d_items);
+    fl->d_items = nullptr;
+    fl->items_cap = 0;
+    if (hipMalloc(&fl->d_items, items.size() * sizeof(WorkItem)) != hipSuccess)
+      return fail(A3D_ERR_NOMEM, "fleet: work list allocation");
+    fl->items_cap = items.size();
+  }
+  A3D_HIP(hipMemcpyAsync(fl->d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                         fl->stream));
+  A3D_HIP(hipStreamSynchronize(fl->stream));  // host vectors go out of scope
+  fl->nitems = (uint32_t)items.size();
+  fl->st_m.assign(m, m + fl->F);
+  fl->staged = true;
+  return A3D_OK;
+}
+
+int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent) {
+  if (!fl || !g) return fail(A3D_ERR_ARG, "a3d_fleet_update_staged: null argument");
+  if (!fl->staged) return fail(A3D_ERR_STATE, "a3d_fleet_update_staged: call a3d_fleet_stage_update first");
+  auto ev = fl->timing.take();
+  if (ev.first) (void)hipEventRecord(ev.first, fl->stream);
+  if (persistent >= 0) {
+    launch_weight(g->g, fl->device_filters(), fl->d_items, fl->nitems, fl->d_counter, current_ppw(), persistent,
+                  fl->stream);
+  } else {
+    // the plain single-filter kernel, launched once per filter
+    for (uint32_t f = 0; f < fl->F; ++f)
+      launch_weight(g->g, single(fl->h_filters[f]), nullptr, 0, nullptr, current_ppw(), 0, fl->stream);
+  }
+  if (ev.first) {
+    (void)hipEventRecord(ev.second, fl->stream);
+    fl->timing.pending.push_back(ev);
+    fl->timing.launches++;
+    uint64_t e = 0;
+    for (uint32_t f = 0; f < fl->F; ++f) e += (uint64_t)fl->n[f] * fl->st_m[f];
+    fl->timing.evals += e;
+  }
+  if (persistent >= 0) {
+    launch_update_tail(g->g, fl->device_filters(), fl->stream);
+  } else {
+    for (uint32_t f = 0; f < fl->F; ++f) launch_update_tail(g->g, single(fl->h_filters[f]), fl->stream);
+  }
+  A3D_HIP(hipGetLastError());
+  return A3D_OK;
+}
+
+int a3d_fleet_update(a3d_fleet* fl, const a3d_grid* g, const float* x, const float* y, const float* z,
+                     const uint32_t* m, const float* ranges, const uint32_t* nb, const double* alpha,
+                     const double* sigma, const double* roll, const double* pitch) {
+  if (!g) return fail(A3D_ERR_ARG, "a3d_fleet_update: null grid");
+  int rc = a3d_fleet_stage_update(fl, x, y, z, m, ranges, nb, alpha, sigma, roll, pitch);
+  if (rc) return rc;
+  rc = a3d_fleet_update_staged(fl, g, 1);
+  if (rc) return rc;
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_resample(a3d_fleet* fl, const float* u01, int mode) {
+  if (!fl || !u01) return fail(A3D_ERR_ARG, "a3d_fleet_resample: null argument");
+  if (mode != A3D_RESAMPLE_REFERENCE && mode != A3D_RESAMPLE_FP64)
+    return fail(A3D_ERR_ARG, "a3d_fleet_resample: unknown mode");
+  for (uint32_t f = 0; f < fl->F; ++f) {
+    FilterDev d = fl->h_filters[f].x ? fl->h_filters[f] : fl->base(f);
+    d.u01 = u01[f];
+    d.idx_out = nullptr;
+    fl->h_filters[f] = d;
+  }
+  int rc = upload_filters(fl);
+  if (rc) return rc;
+  launch_resample(fl->device_filters(), mode, fl->stream);
+  A3D_HIP(hipGetLastError());
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_get_means(const a3d_fleet* fl, float* means4) {
+  if (!fl || !means4) return fail(A3D_ERR_ARG, "a3d_fleet_get_means: null argument");
+  A3D_HIP(hipMemcpyAsync(means4, fl->mean, 16 * (size_t)fl->F, hipMemcpyDeviceToHost, fl->stream));
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_synchronize(a3d_fleet* fl) {
+  if (!fl) return fail(A3D_ERR_ARG, "a3d_fleet_synchronize: null fleet");
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  return A3D_OK;
+}
+
+int a3d_fleet_timing(a3d_fleet* fl, int reset, double* weight_ms_avg, uint64_t* launches, uint64_t* evals) {
+  if (!fl) return fail(A3D_ERR_ARG, "a3d_fleet_timing: null fleet");
+  A3D_HIP(hipStreamSynchronize(fl->stream));
+  fl->timing.resolve();
+  if (weight_ms_avg) *weight_ms_avg = fl->timing.launches ? fl->timing.acc_ms / fl->timing.launches : 0.0;
+  if (launches) *launches = fl->timing.launches;
+  if (evals) *evals = fl->timing.evals;
+  if (reset) {
+    fl->timing.acc_ms = 0.0;
+    fl->timing.launches = 0;
+    fl->timing.evals = 0;
+  }
+  return A3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+// Cloud-weight kernel: Grid3d::computeCloudWeight (Grid3d.cpp:133-199) for
+// every particle of one or many filters, plus the range-beacon weight
+// (ParticleFilter.cpp:224-244) of ParticleFilter::update's first loop
+// (ParticleFilter.cpp:129-153).
+//
+// Mapping.  One wavefront owns a block of up to PB particles of one filter
+// and walks the whole cloud in chunks of 64 points, lane j holding point
+// c0+j.  For each particle i of the block the 64 lanes transform their points
+// with particle i's pose (uniform values taken from lane i with readlane) and
+// gather prob from the grid: 64 points of one particle per gather, so the
+// addresses of one instruction are as close in the map as the points are in
+// the cloud.  The 64 values land in row i of a per-wave LDS tile (0 for a
+// point outside the map).  Lane i then adds its row in point order, so each
+// particle's sum is the reference's sequential float sum over the cloud,
+// bit for bit (adding +0.0f for a rejected point leaves a non-negative sum
+// unchanged).  The count of accepted points comes from a ballot.
+#include "a3d_math.h"
+
+namespace a3d {
+namespace {
+
+constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16-byte rows)
+constexpr int kWaves = 4;      // wavefronts per workgroup
+constexpr int kThreads = 64 * kWaves;
+
+template <bool RP0, bool IDX64>
+__device__ __forceinline__ float gather_one(const GridDev& g, float px, float py, float pz, bool pv, const float* r,
+                                            double ox, double oy, double oz, bool& valid) {
+  float sx, sy, sz;
+  if (RP0) {
+    // roll = pitch = 0: r02 = r12 = +-0, r20 = r21 = +-0, r22 = 1, so the
+    // reference's ((px*r00 + py*r01) + pz*r02) is (px*r00 + py*r01) and the
+    // z sum is pz (finite coordinates; non-finite points are rejected).
+    sx = px * r[0] + py * r[1];
+    sy = px * r[3] + py * r[4];
+    sz = pz;
+  } else {
+    sx = px * r[0] + py * r[1] + pz * r[2];
+    sy = px * r[3] + py * r[4] + pz * r[5];
+    sz = px * r[6] + py * r[7] + pz * r[8];
+  }
+  uint32_t ix, iy, iz;
+  valid = pv && axis_cell(sx, ox, g.ext_hi[0], g.res, g.inv_res, g.size[0], ix) &&
+          axis_cell(sy, oy, g.ext_hi[1], g.res, g.inv_res, g.size[1], iy) &&
+          axis_cell(sz, oz, g.ext_hi[2], g.res, g.inv_res, g.size[2], iz);
+  float v = 0.f;
+  if (valid) {
+    if (IDX64) {
+      const uint64_t idx = (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
+      v = g.prob[idx];
+    } else {
+      const uint32_t idx = ix + iy * g.size[0] + iz * (uint32_t)g.step_z;
+      v = g.prob[idx];
+    }
+  }
+  return v;
+}
+
+// Evaluates the 64 points held by the lanes for every particle of the block
+// whose bit is set in `active`, writing tile rows and accumulating counts.
+template <bool RP0, bool IDX64>
+__device__ __forceinline__ void eval_chunk(const GridDev& g, float px, float py, float pz, bool pv,
+                                           const float (&r)[9], double ox, double oy, double oz, uint32_t count,
+                                           uint64_t active, float* tile, int& cnt) {
+  const int lane = lane_id();
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!((active >> i) & 1ull)) continue;
+    float ri[9];
+    if (RP0) {
+      ri[0] = readlane_f(r[0], i);
+      ri[1] = readlane_f(r[1], i);
+      ri[3] = readlane_f(r[3], i);
+      ri[4] = readlane_f(r[4], i);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) ri[k] = readlane_f(r[k], i);
+    }
+    const double oxi = readlane_d(ox, i), oyi = readlane_d(oy, i), ozi = readlane_d(oz, i);
+    bool valid;
+    const float v = gather_one<RP0, IDX64>(g, px, py, pz, pv, ri, oxi, oyi, ozi, valid);
+    tile[i * kTS + lane] = v;
+    const uint64_t vm = __ballot(valid);
+    cnt += (lane == (int)i) ? (int)__popcll(vm) : 0;
+  }
+}
+
+// KFILTER: particles of a filter (skip those outside the map, write wp and
+// wr); otherwise explicit poses (weight every pose, write out[]).
+template <int PB, bool IDX64, bool KFILTER>
+__device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* tile,
+                             float* out) {
+  const int lane = lane_id();
+  const bool has = (uint32_t)lane < count;
+  const uint32_t pi = p0 + (has ? (uint32_t)lane : 0u);
+  float tx = 0.f, ty = 0.f, tz = 0.f, ta = 0.f;
+  if (has) {
+    tx = f.x[pi];
+    ty = f.y[pi];
+    tz = f.z[pi];
+    ta = f.a[pi];
+  }
+  const bool act = KFILTER ? (has && into_map(g, tx, ty, tz)) : has;
+  const uint64_t active = __ballot(act);
+  if (active == 0) return;
+  float r[9];
+  rotation9(f.roll, f.pitch, ta, r);
+  const double ox = (double)tx - g.min[0];
+  const double oy = (double)ty - g.min[1];
+  const double oz = (double)tz - g.min[2];
+  float wr = 0.f;
+  if (KFILTER && act) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
+
+  float wsum = 0.f;
+  int cnt = 0;
+  const uint32_t m = f.m;
+  const float error_z = 0.f;
+  for (uint32_t c0 = 0; c0 < m; c0 += 64) {
+    const uint32_t j = c0 + (uint32_t)lane;
+    bool pv = j < m;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (pv) {
+      px = f.cx[j];
+      py = f.cy[j];
+      pz = f.cz[j] + error_z;
+    }
+    if (f.rp0) {
+      pv = pv && isfinite(px) && isfinite(py) && isfinite(pz);
+      eval_chunk<true, IDX64>(g, px, py, pz, pv, r, ox, oy, oz, count, active, tile, cnt);
+    } else {
+      eval_chunk<false, IDX64>(g, px, py, pz, pv, r, ox, oy, oz, count, active, tile, cnt);
+    }
+    wave_lds_sync();
+    if (has) {
+      const float4* row = reinterpret_cast<const float4*>(tile + lane * kTS);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const float4 v = row[k];
+        wsum += v.x;
+        wsum += v.y;
+        wsum += v.z;
+        wsum += v.w;
+      }
+    }
+    wave_lds_sync();
+  }
+  if (act) {
+    const float wp = (cnt <= 10) ? 0.f : wsum / (float)cnt;
+    if (KFILTER) {
+      f.wp[pi] = wp;
+      f.wr[pi] = wr;
+    } else {
+      out[pi] = wp;
+    }
+  }
+}
+
+template <int PB, bool IDX64>
+__global__ __launch_bounds__(kThreads) void weight_single_kernel(GridDev g, FilterDev f) {
+  __shared__ float4 tiles4[kWaves * PB * kTS / 4];
+  float* tile = reinterpret_cast<float*>(tiles4) + (threadIdx.x >> 6) * PB * kTS;
+  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t p0 = wave * PB;
+  if (p0 >= f.n) return;
+  const uint32_t count = min((uint32_t)PB, f.n - p0);
+  weight_block<PB, IDX64, true>(g, f, p0, count, tile, nullptr);
+}
+
+template <int PB, bool IDX64>
+__global__ __launch_bounds__(kThreads) void weight_poses_kernel(GridDev g, FilterDev f, float* out) {
+  __shared__ float4 tiles4[kWaves * PB * kTS / 4];
+  float* tile = reinterpret_cast<float*>(tiles4) + (threadIdx.x >> 6) * PB * kTS;
+  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t p0 = wave * PB;
+  if (p0 >= f.n) return;
+  const uint32_t count = min((uint32_t)PB, f.n - p0);
+  weight_block<PB, IDX64, false>(g, f, p0, count, tile, out);
+}
+
+// Many filters: work items (filter, first particle, count) are taken either
+// one per wavefront (grid sized to the list) or, when persistent, from an
+// atomic counter by resident wavefronts until the list is exhausted.  Items
+// are listed in decreasing cost, so the persistent form is a
+// longest-processing-time-first schedule.
+template <int PB, bool IDX64>
+__global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const FilterDev* fs,
+                                                                const WorkItem* items, uint32_t nitems,
+                                                                uint32_t* counter, int persistent) {
+  __shared__ float4 tiles4[kWaves * PB * kTS / 4];
+  float* tile = reinterpret_cast<float*>(tiles4) + (threadIdx.x >> 6) * PB * kTS;
+  const int lane = lane_id();
+  uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (!persistent) {
+    if (k >= nitems) return;
+    const WorkItem it = items[k];
+    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, it.count, tile, nullptr);
+    return;
+  }
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= nitems) break;
+    const WorkItem it = items[t];
+    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, it.count, tile, nullptr);
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+template <int PB>
+void launch_single_pb(const GridDev& g, const FilterDev& f, hipStream_t s) {
+  const uint32_t waves = (f.n + PB - 1) / PB;
+  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  if (g.idx64)
+    weight_single_kernel<PB, true><<<blocks, kThreads, 0, s>>>(g, f);
+  else
+    weight_single_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, f);
+}
+
+template <int PB>
+void launch_poses_pb(const GridDev& g, const FilterDev& f, float* out, hipStream_t s) {
+  const uint32_t waves = (f.n + PB - 1) / PB;
+  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  if (g.idx64)
+    weight_poses_kernel<PB, true><<<blocks, kThreads, 0, s>>>(g, f, out);
+  else
+    weight_poses_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, f, out);
+}
+
+template <int PB>
+void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* items, uint32_t nitems,
+                     uint32_t* counter, int persistent, hipStream_t s) {
+  uint32_t blocks;
+  if (persistent) {
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (g.idx64)
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, weight_fleet_kernel<PB, true>, kThreads, 0);
+    else
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, weight_fleet_kernel<PB, false>, kThreads, 0);
+    if (per_cu < 1) per_cu = 1;
+    blocks = (uint32_t)(cus * per_cu);
+    const uint32_t need = (nitems + kWaves - 1) / kWaves;
+    if (blocks > need) blocks = need;
+    if (blocks == 0) blocks = 1;
+    zero_u32_kernel<<<1, 64, 0, s>>>(counter, 1);
+  } else {
+    blocks = (nitems + kWaves - 1) / kWaves;
+  }
+  if (blocks == 0) return;
+  if (g.idx64)
+    weight_fleet_kernel<PB, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else
+    weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+}
+
+}  // namespace
+
+int clamp_ppw(int ppw) {
+  if (ppw <= 16) return 16;
+  if (ppw <= 32) return 32;
+  return 64;
+}
+
+void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
+                   uint32_t* d_counter, int ppw, int persistent, hipStream_t s) {
+  ppw = clamp_ppw(ppw);
+  if (!fs.arr) {
+    if (fs.one.n == 0) return;
+    if (ppw == 16) launch_single_pb<16>(g, fs.one, s);
+    else if (ppw == 32) launch_single_pb<32>(g, fs.one, s);
+    else launch_single_pb<64>(g, fs.one, s);
+    return;
+  }
+  if (nitems == 0) return;
+  if (ppw == 16) launch_fleet_pb<16>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+  else if (ppw == 32) launch_fleet_pb<32>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+  else launch_fleet_pb<64>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+}
+
+void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, const float* cz, uint32_t m,
+                         const float* tx, const float* ty, const float* tz, const float* yaw, uint32_t n, float roll,
+                         float pitch, float* out, int ppw, hipStream_t s) {
+  if (n == 0) return;
+  FilterDev f{};
+  f.x = const_cast<float*>(tx);
+  f.y = const_cast<float*>(ty);
+  f.z = const_cast<float*>(tz);
+  f.a = const_cast<float*>(yaw);
+  f.n = n;
+  f.cx = cx;
+  f.cy = cy;
+  f.cz = cz;
+  f.m = m;
+  f.roll = roll;
+  f.pitch = pitch;
+  f.rp0 = (roll == 0.0f && pitch == 0.0f) ? 1u : 0u;
+  ppw = clamp_ppw(ppw);
+  if (ppw == 16) launch_poses_pb<16>(g, f, out, s);
+  else if (ppw == 32) launch_poses_pb<32>(g, f, out, s);
+  else launch_poses_pb<64>(g, f, out, s);
+}
+
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  zero_u32_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+}
+
+}  // namespace a3d

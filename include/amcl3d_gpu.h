@@ -1,0 +1,240 @@
+/*
+ * amcl3d_gpu.h -- C ABI of the MI355X (gfx950) particle-filter measurement
+ * update.  The library behind it is hand-written HIP; this header is plain C.
+ *
+ * It replaces the C++ classes of the reference's library target `amcl3d`
+ * (amcl3d/CMakeLists.txt:44-52) on the path the ROS node calls
+ * (amcl3d/src/Node.cpp:41,154,162,167,181,387):
+ *
+ *   Grid3d         amcl3d/src/Grid3d.h:34-159
+ *   ParticleFilter amcl3d/src/ParticleFilter.h:67-213
+ *   computeGrid    amcl3d/src/PointCloudTools.h:96 (PointCloudTools.cpp:84-149)
+ *
+ * Conventions
+ *  - Every function returns 0 (A3D_OK) or an A3D_ERR_* code; a3d_last_error()
+ *    gives the message of the last failure on the calling thread.  No C++
+ *    exception crosses this boundary.
+ *  - Handles own device memory.  Host arrays passed in are read during the
+ *    call only.  Functions taking host arrays are synchronous.  Functions with
+ *    the `_dev` suffix take device pointers (from a3d_buffer_alloc), enqueue
+ *    on the handle's HIP stream and return without waiting.
+ *  - A handle is not thread-safe.  A grid is read-only after it is built and
+ *    may be shared by any number of filters, fleets and servers.
+ *  - Random draws are inputs: standard normals z (4 per particle, order
+ *    x,y,z,yaw) and one uniform u01 per resample.  A draw with standard
+ *    deviation s is (float)(z * (float)s), the value normal_distribution<float>
+ *    produces from z (ParticleFilter.cpp:246-256).
+ *  - Particles are read and written as the reference's struct Particle
+ *    {x, y, z, a, w, wp, wr}: 7 floats, 28 bytes (ParticleFilter.h:35-49).
+ *    Ranges are the reference's struct Range {r, ax, ay, az}: 4 floats
+ *    (ParticleFilter.h:53-63).  Clouds are three float arrays x, y, z (the
+ *    coordinates of pcl::PointXYZ, in cloud order).
+ */
+#ifndef AMCL3D_GPU_H
+#define AMCL3D_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  A3D_OK = 0,
+  A3D_ERR_ARG = 1,    /* invalid argument (null pointer, size 0, bad mode) */
+  A3D_ERR_HIP = 2,    /* HIP runtime failure, or no usable gfx950 device */
+  A3D_ERR_STATE = 3,  /* call not valid in the handle's state */
+  A3D_ERR_NOMEM = 4,  /* device allocation failed / over the HBM budget */
+  A3D_ERR_IO = 5,     /* file could not be read or written */
+  A3D_ERR_BUSY = 6    /* serving: no slot free for the request right now */
+};
+
+enum {
+  A3D_RESAMPLE_REFERENCE = 0, /* float running sum, ParticleFilter.cpp:198-222 */
+  A3D_RESAMPLE_FP64 = 1       /* FP64 prefix sum: sequential double sums in blocks
+                                 of 1024 weights, block totals summed in order;
+                                 u_m = (u01 + m) / N in double */
+};
+
+typedef struct a3d_grid a3d_grid;   /* Grid3d: device-resident likelihood grid */
+typedef struct a3d_pf a3d_pf;       /* ParticleFilter: device-resident particles */
+typedef struct a3d_fleet a3d_fleet; /* many independent filters updated per launch */
+typedef struct a3d_graph a3d_graph; /* a captured chain of _dev calls */
+typedef struct a3d_server a3d_server;
+
+/* ---------------------------------------------------------------- runtime */
+const char* a3d_last_error(void);
+const char* a3d_version(void);
+/* Selects the current HIP device for subsequent handle creation. */
+int a3d_set_device(int device);
+/* Name, CU count and HBM bytes of the current device. */
+int a3d_device_info(char* name, size_t name_len, int* cu_count, size_t* hbm_bytes);
+int a3d_device_synchronize(void);
+
+/* Device buffers for inputs kept resident in HBM (clouds, draws, ranges). */
+int a3d_buffer_alloc(size_t bytes, void** dptr);
+int a3d_buffer_free(void* dptr);
+int a3d_buffer_upload(void* dptr, const void* host, size_t bytes);
+int a3d_buffer_download(void* host, const void* dptr, size_t bytes);
+
+/* ------------------------------------------------------------------- Grid3d */
+/* Grid3d::open minus the octomap/.grid handling (Grid3d.cpp:25-78): builds the
+ * grid on the device with computeGrid (PointCloudTools.cpp:84-149) from the
+ * map's occupied points xyz[3n].  Sizes are ceil((max-min)/res) per axis; the
+ * reference's 250M-cell cap is lifted (64-bit cell indices).  keep_dist != 0
+ * also keeps the squared-distance field (Grid3dCell::dist). */
+int a3d_grid_create_from_points(const float* xyz, size_t n, const double min[3], const double max[3], double res,
+                                double sensor_dev, int keep_dist, a3d_grid** out);
+/* Grid3d::loadGrid without the file (Grid3d.cpp:239-275): cells given by the
+ * caller, x fastest.  size[] must equal ceil((max-min)/res).  dist may be NULL. */
+int a3d_grid_create_from_cells(const float* prob, const float* dist, const uint32_t size[3], const double min[3],
+                               const double max[3], double res, double sensor_dev, a3d_grid** out);
+/* Grid3d::loadGrid / saveGrid with the reference's .grid file layout
+ * (Grid3d.cpp:210-275): u32 sx, sy, sz, f64 sensor_dev, then sx*sy*sz cells of
+ * {f32 dist, f32 prob}.  Bounds and resolution are not in the file and are
+ * given by the caller, as the reference takes them from the octomap.
+ * Loading fails (A3D_ERR_STATE) when |file sensor_dev - sensor_dev| >= DBL_EPSILON. */
+int a3d_grid_load(const char* path, const double min[3], const double max[3], double res, double sensor_dev,
+                  a3d_grid** out);
+int a3d_grid_save(const a3d_grid* g, const char* path);
+int a3d_grid_destroy(a3d_grid* g);
+int a3d_grid_get_info(const a3d_grid* g, uint32_t size[3], double min[3], double max[3], double* res,
+                      double* sensor_dev);
+/* Copies cells [first, first+count) of prob and (if kept and non-NULL) dist. */
+int a3d_grid_read_cells(const a3d_grid* g, uint64_t first, uint64_t count, float* prob, float* dist);
+/* Grid3d::isIntoMap (Grid3d.cpp:201-208). */
+int a3d_grid_is_into_map(const a3d_grid* g, float x, float y, float z, int* inside);
+/* Grid3d::computeCloudWeight (Grid3d.cpp:133-199) for n poses at once:
+ * out[i] = computeCloudWeight(cloud, tx[i], ty[i], tz[i], roll, pitch, yaw[i]). */
+int a3d_grid_cloud_weight(const a3d_grid* g, const float* x, const float* y, const float* z, size_t m,
+                          const float* tx, const float* ty, const float* tz, const float* yaw, size_t n, float roll,
+                          float pitch, float* out);
+
+/* ----------------------------------------------------------- ParticleFilter */
+int a3d_pf_create(a3d_pf** out);
+int a3d_pf_destroy(a3d_pf* pf);
+/* ParticleFilter::init (ParticleFilter.cpp:46-95).  z4: 4*num_particles
+ * standard normals; the first four are unused (particle 0 is the pose). */
+int a3d_pf_init(a3d_pf* pf, int num_particles, float x_init, float y_init, float z_init, float a_init, float x_dev,
+                float y_dev, float z_dev, float a_dev, const float* z4);
+/* ParticleFilter::predict (ParticleFilter.cpp:97-119).  z4: 4*N normals. */
+int a3d_pf_predict(a3d_pf* pf, double odom_x_mod, double odom_y_mod, double odom_z_mod, double odom_a_mod,
+                   double delta_x, double delta_y, double delta_z, double delta_a, const float* z4);
+/* ParticleFilter::update (ParticleFilter.cpp:121-196).  ranges: nb x {r,ax,ay,az}. */
+int a3d_pf_update(a3d_pf* pf, const a3d_grid* g, const float* x, const float* y, const float* z, size_t m,
+                  const float* ranges, size_t nb, double alpha, double sigma, double roll, double pitch);
+/* ParticleFilter::resample (ParticleFilter.cpp:198-222).  idx_out (optional):
+ * N ancestor indices. */
+int a3d_pf_resample(a3d_pf* pf, float u01, int mode, uint32_t* idx_out);
+/* ParticleFilter::getMean (ParticleFilter.h:94-97): {x, y, z, a}. */
+int a3d_pf_get_mean(const a3d_pf* pf, float mean[4]);
+int a3d_pf_is_initialized(const a3d_pf* pf, int* initialized);
+int a3d_pf_size(const a3d_pf* pf, size_t* n);
+/* Particle state as struct Particle[n] (replaces buildParticlesPoseMsg). */
+int a3d_pf_get_particles(const a3d_pf* pf, float* p7, size_t n);
+int a3d_pf_set_particles(a3d_pf* pf, const float* p7, size_t n);
+
+/* Asynchronous variants on device buffers, for inputs resident in HBM and for
+ * graph capture.  d_x, d_y, d_z: cloud arrays; d_ranges: nb x 4 floats. */
+int a3d_pf_predict_dev(a3d_pf* pf, const double mods[4], const double deltas[4], const float* d_z4);
+int a3d_pf_update_dev(a3d_pf* pf, const a3d_grid* g, const float* d_x, const float* d_y, const float* d_z,
+                      size_t m, const float* d_ranges, size_t nb, double alpha, double sigma, double roll,
+                      double pitch);
+int a3d_pf_resample_dev(a3d_pf* pf, float u01, int mode);
+int a3d_pf_synchronize(a3d_pf* pf);
+/* Captures the _dev calls made between begin and end into a HIP graph. */
+int a3d_pf_capture_begin(a3d_pf* pf);
+int a3d_pf_capture_end(a3d_pf* pf, a3d_graph** out);
+int a3d_graph_launch(a3d_graph* graph);
+int a3d_graph_destroy(a3d_graph* graph);
+/* Average duration (ms) of the cloud-weight kernel over the _dev updates made
+ * since the last reset, measured with HIP events on the handle's stream.
+ * Not available inside a capture. */
+int a3d_pf_timing(a3d_pf* pf, int reset, double* weight_ms_avg, uint64_t* launches, uint64_t* evals);
+
+/* Tile shape of the cloud-weight kernel: particles per wavefront (8..64, a
+ * multiple of 8).  Any shape gives identical results. */
+int a3d_set_tile(int particles_per_wave);
+int a3d_get_tile(int* particles_per_wave);
+
+/* -------------------------------------------------------------------- Fleet */
+/* Many independent filters on one grid, updated by one persistent launch.
+ * Per filter f: n[f] particles; all arrays below are concatenations in filter
+ * order, with offsets given by prefix sums of the counts. */
+int a3d_fleet_create(size_t nfilters, const uint32_t* num_particles, a3d_fleet** out);
+int a3d_fleet_destroy(a3d_fleet* fl);
+int a3d_fleet_set_particles(a3d_fleet* fl, const float* p7);
+int a3d_fleet_get_particles(const a3d_fleet* fl, float* p7);
+/* init for every filter: poses/devs: 4 floats per filter. */
+int a3d_fleet_init(a3d_fleet* fl, const float* poses, const float* devs, const float* z4);
+/* mods/deltas: 4 doubles per filter. */
+int a3d_fleet_predict(a3d_fleet* fl, const double* mods, const double* deltas, const float* z4);
+/* Clouds: m[f] points each, concatenated; ranges: nb[f] each, concatenated.
+ * alpha, sigma, roll, pitch: one per filter. */
+int a3d_fleet_update(a3d_fleet* fl, const a3d_grid* g, const float* x, const float* y, const float* z,
+                     const uint32_t* m, const float* ranges, const uint32_t* nb, const double* alpha,
+                     const double* sigma, const double* roll, const double* pitch);
+int a3d_fleet_resample(a3d_fleet* fl, const float* u01, int mode);
+int a3d_fleet_get_means(const a3d_fleet* fl, float* means4);
+/* Same update with clouds, ranges and parameters already on the device
+ * (uploaded once with a3d_fleet_stage_update); runs asynchronously. */
+int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const float* z, const uint32_t* m,
+                           const float* ranges, const uint32_t* nb, const double* alpha, const double* sigma,
+                           const double* roll, const double* pitch);
+int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent);
+int a3d_fleet_synchronize(a3d_fleet* fl);
+/* Cloud-weight kernel timing for the fleet (as a3d_pf_timing). */
+int a3d_fleet_timing(a3d_fleet* fl, int reset, double* weight_ms_avg, uint64_t* launches, uint64_t* evals);
+
+/* ------------------------------------------------------------------ Serving */
+/* A request loop that keeps every session's particles, weights and RNG state
+ * on the device, admits sessions by an HBM budget and batches updates as they
+ * arrive (continuous batching).  Draws come from a counter-based generator
+ * (Philox4x32-10) keyed by the session seed, so a session's results do not
+ * depend on which other sessions share its batches. */
+typedef struct {
+  size_t hbm_budget_bytes;    /* session state + staging admitted at most */
+  uint64_t max_batch_evals;   /* particle.point evaluations per launch */
+  int max_batch_sessions;     /* sessions per launch */
+  int max_inflight;           /* launches in flight (streams) */
+} a3d_server_config;
+
+typedef struct {
+  uint64_t ticket;
+  int session;
+  float mean[4];
+  double latency_ms;    /* submit -> result on the host */
+  double submit_ms;     /* on the server clock */
+  uint64_t evals;
+} a3d_result;
+
+int a3d_server_create(const a3d_server_config* cfg, a3d_server** out);
+int a3d_server_destroy(a3d_server* s);
+/* Adds a grid the sessions may use; the server does not own it. */
+int a3d_server_add_map(a3d_server* s, const a3d_grid* g, int* map_id);
+/* Opens a session (init on the device from the seed).  A3D_ERR_NOMEM when the
+ * HBM budget is exhausted. */
+int a3d_server_open_session(a3d_server* s, int map_id, int num_particles, const float pose[4], const float devs[4],
+                            uint64_t seed, int* session);
+int a3d_server_close_session(a3d_server* s, int session);
+/* Queues predict + update (+ resample if resample != 0) for a session.  The
+ * cloud and ranges are copied at submit. */
+int a3d_server_submit(a3d_server* s, int session, const double mods[4], const double deltas[4], const float* x,
+                      const float* y, const float* z, size_t m, const float* ranges, size_t nb, double alpha,
+                      double sigma, double roll, double pitch, int resample, uint64_t* ticket);
+/* Completed results, oldest first. */
+int a3d_server_poll(a3d_server* s, a3d_result* out, size_t max, size_t* n);
+/* Waits until every submitted request has completed. */
+int a3d_server_drain(a3d_server* s);
+int a3d_server_get_particles(a3d_server* s, int session, float* p7, size_t n);
+/* Counters: requests done, batches launched, evals done, mean batch size. */
+int a3d_server_stats(a3d_server* s, uint64_t* requests, uint64_t* batches, uint64_t* evals, double* busy_ms);
+/* The draws a session's next `count` steps would use (for tests): z4 for
+ * predict (4*N per step) and u01 per step, starting at step `first`. */
+int a3d_session_draws(uint64_t seed, uint64_t step, int num_particles, float* z4, float* u01);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMCL3D_GPU_H */

@@ -1,0 +1,299 @@
+"""GPU parity: the HIP library against the oracle on the same seeded inputs.
+
+Bar: computeGrid's distance field bit-exact and prob bit-exact against the
+oracle's correctly rounded expf (exp_mode=1); per-particle weights and the
+filter state bit-exact (the stated tolerance, 1e-5 relative, is checked too);
+resample indices bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from amcl3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def room(gpu):
+    c = W.room_case(1)
+    bm = c["map"]
+    pts = bm.points()
+    dist, prob = oracle.compute_grid(pts, bm.min, bm.max, bm.res, c["sensor_dev"], exp_mode=1)
+    g = gpu.Grid3d()
+    _say("room: device computeGrid", bm.size, pts.shape)
+    g.open(pts, bm.min, bm.max, bm.res, c["sensor_dev"], keep_dist=True)
+    c.update(points=pts, odist=dist, oprob=prob, grid=g, ogrid=oracle.GridMeta(bm.min, bm.max, bm.res, prob))
+    return c
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+# -------------------------------------------------------------- computeGrid
+def test_grid_room_bit_exact(room):
+    p, d = room["grid"].read_cells(dist=True)
+    assert list(room["grid"].size) == [200, 200, 60]
+    assert np.array_equal(d, room["odist"].reshape(-1))
+    mism = (p.view(np.int32) != room["oprob"].reshape(-1).view(np.int32)).sum()
+    _say("room grid prob mismatches vs oracle:", mism)
+    assert mism == 0
+
+
+def test_grid_kat_map(gpu, kat, kat_oracle_grid):
+    ext = np.asarray(kat["map_max"]) - np.asarray(kat["map_min"])
+    g = gpu.Grid3d()
+    _say("kat: device computeGrid 522x384x153")
+    g.open(kat["map_cloud"], np.zeros(3), ext, kat["map_res"], kat["sensor_dev"], keep_dist=True)
+    prob, dist = g.read_cells(dist=True)
+    odist, oprob = kat_oracle_grid
+    assert np.array_equal(dist, odist.reshape(-1))
+    ulps = np.abs(prob.view(np.int32).astype(np.int64) - oprob.reshape(-1).view(np.int32))
+    _say("kat grid prob: cells differing from glibc expf:", int((ulps != 0).sum()), "max ulp", int(ulps.max()))
+    assert ulps.max() <= 1
+    # KAT-2: the nav slice
+    s = oracle.slice_int8(prob.reshape(odist.shape)[20]).reshape(kat["nav"].shape).astype(np.int32)
+    dd = s - kat["nav"].astype(np.int32)
+    assert np.abs(dd).max() <= 1 and (dd == 0).mean() >= 0.999
+    # KAT-1: computeCloudWeight at the known pose, on the device-built cells
+    g2 = gpu.Grid3d().from_cells(prob, kat["map_min"], kat["map_max"], kat["map_res"], kat["sensor_dev"])
+    p = kat["kat_pose"]
+    w = g2.computeCloudWeight(kat["sensor_cloud"], p["tx"], p["ty"], p["tz"], p["roll"], p["pitch"], p["yaw"])
+    og = oracle.GridMeta(kat["map_min"], kat["map_max"], kat["map_res"], prob)
+    ow = oracle.cloud_weight(og, kat["sensor_cloud"], p["tx"], p["ty"], p["tz"], p["roll"], p["pitch"], p["yaw"])
+    _say("KAT-1 device weight", w, "reference 3.8109049797058105")
+    assert w == ow
+    assert abs(w - kat["kat_weight"]) <= kat["kat_tolerance"]
+    assert not g2.isIntoMap(*kat["isinto_out"]) and g2.isIntoMap(*kat["isinto_in"])
+
+
+def test_grid_empty_and_tiny(gpu):
+    g = gpu.Grid3d()
+    g.open(np.zeros((0, 3), F32), np.zeros(3), np.ones(3), 0.25, 0.1, keep_dist=True)
+    p, d = g.read_cells(dist=True)
+    assert np.all(p == 0) and np.all(d == -1)
+    rng = np.random.default_rng(7)
+    mn = np.array([-1.3, 0.7, -0.35])
+    mx = mn + np.array([1.2, 1.0, 0.8])
+    pts = (mn + rng.uniform(-0.2, 1.3, (40, 3)) * (mx - mn)).astype(F32)
+    pts[3] = np.nan  # invalid points are dropped, as PCL's kd-tree does
+    g.open(pts, mn, mx, 0.1, 0.05, keep_dist=True)
+    od, op = oracle.compute_grid(pts, mn, mx, 0.1, 0.05, exp_mode=1)
+    p, d = g.read_cells(dist=True)
+    assert np.array_equal(d, od.reshape(-1)) and np.array_equal(p, op.reshape(-1))
+
+
+# --------------------------------------------------------- computeCloudWeight
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_cloud_weights_bit_exact(gpu, room, tile):
+    gpu.set_tile(tile)
+    try:
+        s, p = room["step"], room["particles"]
+        w = room["grid"].cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+        ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+        assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+    finally:
+        gpu.set_tile(32)
+
+
+def test_cloud_weights_roll_pitch_and_edges(gpu, room):
+    rng = np.random.default_rng(11)
+    s = room["step"]
+    n = 300
+    p = W.particles_around(room["pose"], n, rng, dxyz=2.0, dyaw=3.0)
+    p[:20, 0] = -3.0  # pose outside the map: still weighted (computeCloudWeight has no map check)
+    for roll, pitch in ((0.05, -0.02), (-0.3, 0.2)):
+        w = room["grid"].cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+    # points exactly on cell faces (yaw 0, lattice translation): the division fallback
+    bm = room["map"]
+    lat = (np.stack(np.meshgrid(np.arange(-20, 20), np.arange(-20, 20), np.arange(-5, 5)), -1).reshape(-1, 3) *
+           bm.res).astype(F32)
+    tx = np.array([10.0, 10.05, 9.95, 0.0, 19.999], F32)
+    ty = np.array([10.0, 10.0, 10.1, 0.0, 0.001], F32)
+    tz = np.array([3.0, 3.0, 2.9, 0.0, 5.999], F32)
+    ya = np.zeros(5, F32)
+    w = room["grid"].cloud_weights(lat, tx, ty, tz, ya)
+    ow = oracle.cloud_weight_batch(room["ogrid"], lat, tx, ty, tz, ya)
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+    # <= 10 in-map points -> 0; nothing in the map -> 0
+    w = room["grid"].cloud_weights(s.cloud[:10], p[:5, 0], p[:5, 1], p[:5, 2], p[:5, 3])
+    assert np.all(w == 0)
+    w = room["grid"].cloud_weights(s.cloud + 100, p[:5, 0], p[:5, 1], p[:5, 2], p[:5, 3])
+    assert np.all(w == 0)
+    # non-finite points are rejected like the reference's NaN compares
+    c = s.cloud.copy()
+    c[::7, 2] = np.nan
+    c[::11, 0] = np.inf
+    w = room["grid"].cloud_weights(c, p[:64, 0], p[:64, 1], p[:64, 2], p[:64, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], c, p[:64, 0], p[:64, 1], p[:64, 2], p[:64, 3])
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
+def test_cloud_weights_64bit_index_path(gpu, room, monkeypatch):
+    monkeypatch.setenv("A3D_FORCE_IDX64", "1")
+    bm = room["map"]
+    g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
+    s, p = room["step"], room["particles"]
+    w = g.cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
+# ---------------------------------------------------------------- the filter
+def test_init_bit_exact(gpu):
+    rng = np.random.default_rng(2)
+    for n in (1, 7, 1000, 5000):
+        z = rng.standard_normal((n, 4)).astype(F32)
+        pf = gpu.ParticleFilter()
+        assert not pf.isInitialized()
+        pf.init(n, 2.87080164308, 1.63820032984, 0.21460881682, 0.144, 0.05, 0.05, 0.05, 0.1, z)
+        assert pf.isInitialized()
+        op, om = oracle.pf_init(n, (2.87080164308, 1.63820032984, 0.21460881682, 0.144), (0.05, 0.05, 0.05, 0.1), z)
+        assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32)), n
+        assert np.array_equal(pf.getMean().view(np.int32), om.view(np.int32)), n
+
+
+def test_predict_keeps_mean_and_is_bit_exact(gpu):
+    # ParticleFilterTest.cpp:39-72: predict does not change getMean()
+    rng = np.random.default_rng(3)
+    n = 600
+    z = rng.standard_normal((n, 4)).astype(F32)
+    pf = gpu.ParticleFilter()
+    pf.init(n, 2.87080164308, 1.63820032984, 0.21460881682, 0.144, 0.05, 0.05, 0.05, 0.1, z)
+    m0 = pf.getMean()
+    before = pf.particles()
+    z2 = rng.standard_normal((n, 4)).astype(F32)
+    pf.predict(0.1, 0.1, 0.1, 0.3, -0.067421, -0.006161, 0.130909, 0.052421, z2)
+    assert np.array_equal(pf.getMean(), m0)
+    op = oracle.pf_predict(before, [0.1, 0.1, 0.1, 0.3], [-0.067421, -0.006161, 0.130909, 0.052421], z2)
+    assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32))
+
+
+def _full_step_case(gpu, room, seed, n=1000, out_of_map=0, nb=3):
+    rng = np.random.default_rng(seed)
+    bm = room["map"]
+    pose = np.append(W.free_pose(bm, rng, z_range=(1.0, 2.5)), rng.uniform(-np.pi, np.pi))
+    cloud = W.sensor_cloud(bm, pose, 2000, 0.05, rng)
+    ranges = W.beacons(bm, pose, nb, 0.1, rng)
+    parts = W.particles_around(pose, n, rng)
+    parts[:out_of_map, 2] = -1.0
+    parts[:, 5] = rng.uniform(0, 1, n)  # stale wp/wr values are part of the state
+    parts[:, 6] = rng.uniform(0, 1, n)
+    z4 = rng.standard_normal((n, 4)).astype(F32)
+    return pose, cloud, ranges, parts, z4, float(rng.uniform())
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5])
+def test_full_step_bit_exact(gpu, room, seed):
+    _say("full step seed", seed)
+    pose, cloud, ranges, parts, z4, u01 = _full_step_case(gpu, room, seed, out_of_map=37 if seed == 2 else 0,
+                                                          nb=0 if seed == 3 else 3)
+    mods, deltas = [0.1, 0.1, 0.1, 0.3], [0.05, -0.03, 0.01, 0.02]
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    pf.predict(*mods, *deltas, z4)
+    pf.update(room["grid"], cloud, ranges, 0.8, 0.1, 0.0, 0.0)
+    got = pf.particles()
+    mean = pf.getMean()
+    op = oracle.pf_predict(parts, mods, deltas, z4)
+    op, om = oracle.pf_update(op, room["ogrid"], cloud, ranges, 0.8, 0.1, 0.0, 0.0)
+    rel = _rel(got[:, 4], op[:, 4])[op[:, 4] > 0]
+    _say("seed", seed, "max rel err of w:", rel.max() if rel.size else 0.0)
+    assert rel.size == 0 or rel.max() <= 1e-5  # the stated tolerance
+    assert np.array_equal(got.view(np.int32), op.view(np.int32))  # and in fact bit-exact
+    assert np.array_equal(mean.view(np.int32), om.view(np.int32))
+    for mode in (gpu.RESAMPLE_REFERENCE, gpu.RESAMPLE_FP64):
+        pf2 = gpu.ParticleFilter()
+        pf2.set_particles(got)
+        idx = pf2.resample(u01, mode, return_indices=True)
+        oq, oidx = oracle.pf_resample(op, u01, mode)
+        assert np.array_equal(idx, oidx), mode
+        assert np.array_equal(pf2.particles().view(np.int32), oq.view(np.int32))
+    ref_idx = oracle.resample_indices(op[:, 4], u01, 0)
+    f64_idx = oracle.resample_indices(op[:, 4], u01, 1)
+    _say("seed", seed, "resample indices differing between float and FP64 prefix:", int((ref_idx != f64_idx).sum()))
+
+
+def test_update_roll_pitch_and_no_map(gpu, room):
+    pose, cloud, ranges, parts, z4, u01 = _full_step_case(gpu, room, 21)
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    pf.update(room["grid"], cloud, ranges, 0.5, 0.53, 0.07, -0.04)
+    op, om = oracle.pf_update(parts, room["ogrid"], cloud, ranges, 0.5, 0.53, 0.07, -0.04)
+    assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32))
+    assert np.array_equal(pf.getMean().view(np.int32), om.view(np.int32))
+    pf.set_particles(parts)
+    pf.update(None, cloud, ranges, 0.5, 0.53)
+    op, om = oracle.pf_update(parts, oracle.GridMeta(room["map"].min, room["map"].max, 0.1, None), cloud, ranges,
+                              0.5, 0.53)
+    assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32))
+    assert np.all(pf.getMean() == 0)
+
+
+def test_large_filter_sampled(gpu, room):
+    # 100,000 particles x 20,000 points: sampled particles against the oracle,
+    # and size-independent properties over all of them
+    rng = np.random.default_rng(77)
+    bm = room["map"]
+    pose = np.append(W.free_pose(bm, rng, z_range=(1.0, 2.5)), 0.3)
+    n, m = 100000, 20000
+    cloud = W.sensor_cloud(bm, pose, m, 0.03, rng)
+    ranges = W.beacons(bm, pose, 8, 0.1, rng)
+    parts = W.particles_around(pose, n, rng)
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    _say("large update n=1e5 m=2e4")
+    pf.update(room["grid"], cloud, ranges, 0.8, 0.1)
+    got = pf.particles()
+    sel = rng.choice(n, 300, replace=False)
+    ow = oracle.cloud_weight_batch(room["ogrid"], cloud, parts[sel, 0], parts[sel, 1], parts[sel, 2],
+                                   parts[sel, 3])
+    # wp after normalisation is wp_raw / wtp: compare the ratios of sampled pairs
+    raw = got[sel, 5].astype(np.float64)
+    ratio = raw / max(raw.max(), 1e-30)
+    oratio = ow.astype(np.float64) / max(ow.max(), 1e-30)
+    assert np.allclose(ratio, oratio, rtol=1e-6, atol=1e-9)
+    assert abs(got[:, 4].astype(np.float64).sum() - 1.0) < 1e-3
+    idx = pf.resample(0.37, gpu.RESAMPLE_REFERENCE, return_indices=True)
+    assert np.all(np.diff(idx.astype(np.int64)) >= 0) and idx.max() < n
+    oidx = oracle.resample_indices(got[:, 4], 0.37, 0)
+    assert np.array_equal(idx, oidx)
+
+
+def test_graph_capture_replays_match_eager(gpu, room):
+    pose, cloud, ranges, parts, z4, u01 = _full_step_case(gpu, room, 31)
+    mods, deltas = [0.1, 0.1, 0.1, 0.3], [0.05, -0.03, 0.01, 0.02]
+    dz = gpu.DeviceBuffer.from_array(z4)
+    x, y, z = (gpu.DeviceBuffer.from_array(np.ascontiguousarray(cloud[:, k])) for k in range(3))
+    dr = gpu.DeviceBuffer.from_array(ranges)
+    eager = gpu.ParticleFilter()
+    eager.set_particles(parts)
+    graphed = gpu.ParticleFilter()
+    graphed.set_particles(parts)
+    graphed.capture_begin()
+    graphed.predict_dev(mods, deltas, dz)
+    graphed.update_dev(room["grid"], x.ptr, y.ptr, z.ptr, cloud.shape[0], dr.ptr, ranges.shape[0], 0.8, 0.1)
+    graphed.resample_dev(u01)
+    graph = graphed.capture_end()
+    for rep in range(3):
+        eager.predict(*mods, *deltas, z4)
+        eager.update(room["grid"], cloud, ranges, 0.8, 0.1)
+        eager.resample(u01)
+        graph.launch()
+        graphed.synchronize()
+        assert np.array_equal(graphed.particles().view(np.int32), eager.particles().view(np.int32)), rep
+        assert np.array_equal(graphed.getMean(), eager.getMean()), rep
+    graph.close()
