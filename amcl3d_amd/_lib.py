@@ -148,7 +148,19 @@ def check(rc):
 
 
 def call(name, *args):
-    check(getattr(load(), name)(*args))
+    """Calls a C-ABI function.  numpy arrays may be passed directly: they are
+    turned into data pointers here, while this frame keeps them alive (a
+    pointer taken from a temporary array would dangle once it is freed)."""
+    conv = []
+    for a in args:
+        if isinstance(a, np.ndarray):
+            if not a.flags["C_CONTIGUOUS"]:
+                raise ValueError("%s: array argument must be C-contiguous" % name)
+            conv.append(a.ctypes.data)
+        else:
+            conv.append(a)
+    check(getattr(load(), name)(*conv))
+    del args
 
 
 def ptr(a):

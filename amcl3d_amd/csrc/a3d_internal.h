@@ -82,7 +82,8 @@ struct FilterDev {
   float pose[4];    // init
   // resample
   float u01;
-  uint32_t* idx_out;  // optional: ancestor indices
+  uint32_t no_resample;  // a batch member that did not ask to resample
+  uint32_t* idx_out;     // optional: ancestor indices
 };
 
 // A unit of cloud-weight work: particles [p0, p0+count) of one filter.
@@ -98,6 +99,7 @@ struct Filters {
   const FilterDev* arr;  // device array, or null
   uint32_t count;        // number of filters
   uint32_t max_n;        // largest particle count
+  const float* u01;      // optional device array: resample uniform of filter i
   FilterDev one;         // used when arr == null
 };
 
@@ -128,6 +130,8 @@ void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s);
 void launch_predict(const Filters& fs, hipStream_t s);
 void launch_init(const Filters& fs, hipStream_t s);
 void launch_resample(const Filters& fs, int mode, hipStream_t s);
+// out[4i .. 4i+3] = mean of filter i
+void launch_gather_means(const Filters& fs, float* out, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 
 // Device map build (computeGrid, PointCloudTools.cpp:84-149).

@@ -231,6 +231,7 @@ __global__ __launch_bounds__(kTailThreads) void resample_prefix_kernel(Filters f
   __shared__ float4 pstage4[2 * kCH / 4];
   __shared__ float res[8];
   const FilterDev& f = fsel(fs, blockIdx.x);
+  if (f.no_resample) return;
   const uint32_t n = f.n;
   float* sc = f.scratch;
   if (mode == A3D_RESAMPLE_REFERENCE) {
@@ -286,17 +287,18 @@ __global__ void resample_select_kernel(Filters fs, int mode) {
   const FilterDev& f = fsel(fs, blockIdx.y);
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = f.n;
-  if (m >= n) return;
+  if (m >= n || f.no_resample) return;
   float* sc = f.scratch;
+  const float u01 = fs.u01 ? fs.u01[blockIdx.y] : f.u01;
   uint32_t i;
   if (mode == A3D_RESAMPLE_REFERENCE) {
     const float factor = 1.f / (float)n;  // 1.f / p_.size() (size_t -> float)
-    const float r = factor * f.u01;
+    const float r = factor * u01;
     const float u = r + factor * (float)m;
     i = first_not_below(sc, n, u);
   } else {
     const double* c64 = reinterpret_cast<const double*>(sc + sc_off(10, n));
-    const double u = ((double)f.u01 + (double)m) * (1.0 / (double)n);
+    const double u = ((double)u01 + (double)m) * (1.0 / (double)n);
     i = first_not_below(c64, n, u);
   }
   if (f.idx_out) f.idx_out[m] = i;
@@ -312,7 +314,7 @@ __global__ void resample_commit_kernel(Filters fs) {
   const FilterDev& f = fsel(fs, blockIdx.y);
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = f.n;
-  if (m >= n) return;
+  if (m >= n || f.no_resample) return;
   const float* sc = f.scratch;
   f.x[m] = sc[sc_off(4, n) + m];
   f.y[m] = sc[sc_off(5, n) + m];
@@ -323,7 +325,19 @@ __global__ void resample_commit_kernel(Filters fs) {
   f.w[m] = 1.f / (float)n;
 }
 
+__global__ void gather_means_kernel(Filters fs, float* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fs.count) return;
+  const FilterDev& f = fsel(fs, i);
+  for (int k = 0; k < 4; ++k) out[4 * (size_t)i + k] = f.mean[k];
+}
+
 }  // namespace
+
+void launch_gather_means(const Filters& fs, float* out, hipStream_t s) {
+  if (fs.count == 0) return;
+  gather_means_kernel<<<(fs.count + 63) / 64, 64, 0, s>>>(fs, out);
+}
 
 void launch_predict(const Filters& fs, hipStream_t s) {
   if (fs.max_n == 0) return;

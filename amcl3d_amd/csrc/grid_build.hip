@@ -108,6 +108,7 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
   __shared__ uint32_t blist[kGThreads];
   __shared__ uint32_t bpre[kGThreads + 1];
   __shared__ uint32_t nlist;
+  __shared__ uint32_t notdone;
   const int tid = threadIdx.x;
   const uint64_t tile = tile0 + blockIdx.x;
   const uint32_t tx = (uint32_t)(tile % ntx);
@@ -139,8 +140,14 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
       hi[a] = (float)(g.min[a] + (double)c1_ * g.res);
     }
   }
+  // A round processes the buckets inside its cube (the bucket ranges covering
+  // the tile box grown by r) whose gap is <= r, minus those the previous
+  // round took by the same test.  Cubes and radii grow, so that union is
+  // exactly what every earlier round processed, and every point within r of
+  // the tile box lies in a processed bucket.
   double r_prev = -1.0;
   double r = b.h;
+  uint32_t plo[3] = {1u, 1u, 1u}, phi[3] = {0u, 0u, 0u};  // previous cube (empty)
   for (;;) {
     uint32_t blo[3], bhi[3];
 #pragma unroll
@@ -163,7 +170,9 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
         const uint32_t cnt = b.start[id + 1] - b.start[id];
         if (cnt) {
           const double gap = box_gap(b, lo, hi, bi);
-          if (gap <= r && gap > r_prev) {
+          const bool in_prev = bi[0] >= plo[0] && bi[0] <= phi[0] && bi[1] >= plo[1] && bi[1] <= phi[1] &&
+                               bi[2] >= plo[2] && bi[2] <= phi[2] && gap <= r_prev;
+          if (gap <= r && !in_prev) {
             const uint32_t slot = atomicAdd(&nlist, 1u);
             blist[slot] = id;
           }
@@ -171,7 +180,10 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
       }
       __syncthreads();
       const uint32_t nl = nlist;
-      if (nl == 0) continue;
+      if (nl == 0) {
+        __syncthreads();  // nlist is reset at the top of the next chunk
+        continue;
+      }
       if (tid == 0) {
         uint32_t acc = 0;
         for (uint32_t k = 0; k < nl; ++k) {
@@ -218,7 +230,19 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
 #pragma unroll
     for (int c = 0; c < 4; ++c)
       if (cv[c] && !((double)best[c] <= lim)) ok = false;
-    if (__syncthreads_and(ok ? 1 : 0) || r >= rmax) break;
+    // workgroup AND of ok through one LDS word (every thread passes each barrier)
+    if (tid == 0) notdone = 0u;
+    __syncthreads();
+    if (!ok) notdone = 1u;
+    __syncthreads();
+    const bool finished = (notdone == 0u) || r >= rmax;
+    __syncthreads();
+    if (finished) break;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      plo[a] = blo[a];
+      phi[a] = bhi[a];
+    }
     r_prev = r;
     r = r * 2.0;
   }

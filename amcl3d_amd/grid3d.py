@@ -30,7 +30,7 @@ class Grid3d:
         pts = _lib.f32(points).reshape(-1, 3)
         h = ctypes.c_void_p()
         _lib.call("a3d_grid_create_from_points", _lib.ptr(pts) if pts.shape[0] else None, pts.shape[0],
-                  _lib.ptr(_lib.f64(min_xyz)), _lib.ptr(_lib.f64(max_xyz)), float(res), float(sensor_dev),
+                  _lib.f64(min_xyz), _lib.f64(max_xyz), float(res), float(sensor_dev),
                   1 if keep_dist else 0, ctypes.byref(h))
         self._h = h.value
         self._read_info()
@@ -44,8 +44,8 @@ class Grid3d:
         size = np.ceil((np.asarray(max_xyz, np.float64) - np.asarray(min_xyz, np.float64)) / res)
         sz = _lib.u32(size)
         h = ctypes.c_void_p()
-        _lib.call("a3d_grid_create_from_cells", _lib.ptr(p), _lib.ptr(d), _lib.ptr(sz), _lib.ptr(_lib.f64(min_xyz)),
-                  _lib.ptr(_lib.f64(max_xyz)), float(res), float(sensor_dev), ctypes.byref(h))
+        _lib.call("a3d_grid_create_from_cells", _lib.ptr(p), _lib.ptr(d), _lib.ptr(sz), _lib.f64(min_xyz),
+                  _lib.f64(max_xyz), float(res), float(sensor_dev), ctypes.byref(h))
         self._h = h.value
         self._read_info()
         return self
@@ -54,7 +54,7 @@ class Grid3d:
         """Grid3d::loadGrid (Grid3d.cpp:239-275): the reference's .grid file."""
         self.close()
         h = ctypes.c_void_p()
-        _lib.call("a3d_grid_load", str(path).encode(), _lib.ptr(_lib.f64(min_xyz)), _lib.ptr(_lib.f64(max_xyz)),
+        _lib.call("a3d_grid_load", str(path).encode(), _lib.f64(min_xyz), _lib.f64(max_xyz),
                   float(res), float(sensor_dev), ctypes.byref(h))
         self._h = h.value
         self._read_info()

@@ -95,7 +95,7 @@ class ParticleFilter:
 
     # -- resident-input (device buffer) forms, graph capture ---------------
     def predict_dev(self, mods, deltas, d_z4):
-        _lib.call("a3d_pf_predict_dev", self._h, _lib.ptr(_lib.f64(mods)), _lib.ptr(_lib.f64(deltas)), d_z4.ptr)
+        _lib.call("a3d_pf_predict_dev", self._h, _lib.f64(mods), _lib.f64(deltas), d_z4.ptr)
 
     def update_dev(self, grid3d, d_x, d_y, d_z, m, d_ranges, nb, alpha, sigma, roll=0.0, pitch=0.0):
         _lib.call("a3d_pf_update_dev", self._h, grid3d.handle, d_x, d_y, d_z, int(m), d_ranges, int(nb),

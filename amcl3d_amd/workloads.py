@@ -66,8 +66,13 @@ class BoxMap:
         t_best = np.full(d.shape[0], np.inf)
         with np.errstate(divide="ignore", invalid="ignore"):
             inv = 1.0 / d
-        for c0 in range(0, self.boxes.shape[0], 64):
-            bx = self.boxes[c0:c0 + 64]
+        # only boxes that a ray of length max_range can reach
+        lo = self.boxes[:, :3]
+        hi = self.boxes[:, 3:]
+        gap = np.maximum(np.maximum(lo - o, o - hi), 0.0)
+        near = self.boxes[np.sum(gap * gap, 1) <= max_range * max_range]
+        for c0 in range(0, near.shape[0], 64):
+            bx = near[c0:c0 + 64]
             t1 = (bx[None, :, :3] - o) * inv[:, None, :]
             t2 = (bx[None, :, 3:] - o) * inv[:, None, :]
             tmin = np.nanmax(np.minimum(t1, t2), -1)
@@ -124,6 +129,50 @@ def building_map(seed, extent=(100.0, 100.0, 30.0), res=0.05, floor_height=3.0, 
         c[2] = k * floor_height + t + side[2] / 2
         boxes.append(list(c - side / 2) + list(c + side / 2))
     return BoxMap(np.zeros(3), ex, res, boxes)
+
+
+def storey_view(bmap, z, floor_height):
+    """The boxes a sensor at height z sees: its storey's floor, ceiling, walls, obstacles."""
+    k = np.floor(z / floor_height)
+    z0, z1 = k * floor_height - 0.01, (k + 1) * floor_height + 0.01
+    keep = (bmap.boxes[:, 5] >= z0) & (bmap.boxes[:, 2] <= z1)
+    return BoxMap(bmap.min, bmap.max, bmap.res, bmap.boxes[keep])
+
+
+def trajectory(bmap, steps, seed, z=1.5, speed=0.25, floor_height=3.0):
+    """A smooth seeded path in free space: (steps+1, 4) poses x, y, z, yaw, moving
+    about `speed` m and turning a few degrees per step."""
+    rng = np.random.default_rng(seed)
+    view = storey_view(bmap, z, floor_height)
+    for _ in range(200):
+        start = free_pose(view, rng, z_range=(z, z), clearance=1.0)
+        poses = [np.append(start, rng.uniform(-np.pi, np.pi))]
+        turn = rng.uniform(-0.05, 0.05)
+        ok = True
+        for _k in range(steps):
+            x, y, zz, a = poses[-1]
+            for _try in range(20):
+                a2 = a + turn
+                nxt = np.array([x + speed * np.cos(a2), y + speed * np.sin(a2), zz, a2])
+                if not view.inside_any(nxt[:3], 0.8)[0] and np.all(nxt[:2] > bmap.min[:2] + 1) and \
+                        np.all(nxt[:2] < bmap.max[:2] - 1):
+                    break
+                turn = rng.uniform(-0.6, 0.6)
+            else:
+                ok = False
+                break
+            poses.append(nxt)
+            turn = 0.8 * turn + rng.uniform(-0.02, 0.02)
+        if ok:
+            return np.array(poses)
+    raise RuntimeError("no free trajectory found")
+
+
+def odom_delta(p0, p1):
+    """Odometry increment from pose p0 to p1 in p0's frame (Node.cpp:147-151)."""
+    c, s = np.cos(p0[3]), np.sin(p0[3])
+    dx, dy = p1[0] - p0[0], p1[1] - p0[1]
+    return np.array([c * dx + s * dy, -s * dx + c * dy, p1[2] - p0[2], p1[3] - p0[3]])
 
 
 def free_pose(bmap, rng, z_range=None, clearance=0.6):

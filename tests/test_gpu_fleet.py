@@ -49,13 +49,13 @@ def test_fleet_matches_single_filters(gpu):
     R = np.concatenate(rngs) if sum(nbs) else np.zeros((0, 4), F32)
     fl = _fleet(gpu, counts)
     try:
-        _lib.call("a3d_fleet_set_particles", fl, _lib.ptr(_lib.f32(P)))
+        _lib.call("a3d_fleet_set_particles", fl, _lib.f32(P))
         x, y, z = _lib.cloud_soa(cl)
-        args = [_lib.ptr(x), _lib.ptr(y), _lib.ptr(z), _lib.ptr(_lib.u32(ms)), _lib.ptr(_lib.f32(R)),
-                _lib.ptr(_lib.u32(nbs)), _lib.ptr(_lib.f64(alpha)), _lib.ptr(_lib.f64(sigma)),
-                _lib.ptr(_lib.f64(roll)), _lib.ptr(_lib.f64(pitch))]
+        args = [_lib.ptr(x), _lib.ptr(y), _lib.ptr(z), _lib.u32(ms), _lib.f32(R),
+                _lib.u32(nbs), _lib.f64(alpha), _lib.f64(sigma),
+                _lib.f64(roll), _lib.f64(pitch)]
         for persistent in (1, 0, -1):
-            _lib.call("a3d_fleet_set_particles", fl, _lib.ptr(_lib.f32(P)))
+            _lib.call("a3d_fleet_set_particles", fl, _lib.f32(P))
             _lib.call("a3d_fleet_stage_update", fl, *args)
             _lib.call("a3d_fleet_update_staged", fl, grid.handle, persistent)
             _lib.call("a3d_fleet_synchronize", fl)

@@ -1,0 +1,102 @@
+"""GPU: served sessions give the same result alone and in the mix, and the
+same as the plain ParticleFilter driven with the session's draws."""
+import numpy as np
+import pytest
+
+from amcl3d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def room_grid(gpu):
+    bm = W.room_map(1)
+    g = gpu.Grid3d()
+    g.open(bm.points(), bm.min, bm.max, bm.res, 0.1)
+    return bm, g
+
+
+def _sessions(bm, k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(k):
+        n = int(rng.integers(50, 3000))
+        m = int(rng.integers(20, 2500))
+        pose = np.append(W.free_pose(bm, rng, z_range=(1.0, 2.5)), rng.uniform(-3, 3)).astype(F32)
+        steps = []
+        for _ in range(3):
+            c = W.sensor_cloud(bm, pose, m, 0.05, rng)
+            steps.append((tuple(np.ascontiguousarray(c[:, j]) for j in range(3)), W.beacons(bm, pose, 3, 0.1, rng),
+                          rng.uniform(-0.1, 0.1, 4), bool(rng.integers(0, 2))))
+        out.append((n, pose, 1000 + s, steps))
+    return out
+
+
+def _run(gpu, grid, sessions, alone):
+    from amcl3d_amd.serving import Server
+
+    srv = Server(max_inflight=1 if alone else 3, max_batch_sessions=1 if alone else 64)
+    mid = srv.add_map(grid)
+    ids = [srv.open_session(mid, n, pose, [0.3, 0.3, 0.1, 0.2], seed) for n, pose, seed, _ in sessions]
+    mods = [0.1, 0.1, 0.1, 0.3]
+    for k in range(3):
+        for sid, (_, _, _, steps) in zip(ids, sessions):
+            xyz, rg, d, rs = steps[k]
+            srv.submit(sid, mods, d, xyz, rg, 0.8, 0.1, resample=rs)
+            if alone:
+                srv.drain()
+    srv.drain()
+    res = srv.poll(100000)
+    out = [srv.particles(sid) for sid in ids]
+    st = srv.stats()
+    srv.close()
+    return out, res, st
+
+
+def test_same_alone_and_in_the_mix(gpu, room_grid):
+    bm, grid = room_grid
+    sessions = _sessions(bm, 10, 4)
+    alone, res_a, _ = _run(gpu, grid, sessions, alone=True)
+    mixed, res_m, st = _run(gpu, grid, sessions, alone=False)
+    assert len(res_a) == len(res_m) == 30
+    assert st["batches"] < 30  # requests were batched together
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    ma = {(r.session, r.ticket % 1000000): np.array(r.mean) for r in res_a}
+    assert all(np.isfinite(r.latency_ms) and r.latency_ms >= 0 for r in res_m)
+    assert len(ma) == 30
+
+
+def test_served_equals_plain_filter(gpu, room_grid):
+    from amcl3d_amd.serving import session_draws
+
+    bm, grid = room_grid
+    sessions = _sessions(bm, 3, 9)
+    served, _, _ = _run(gpu, grid, sessions, alone=False)
+    mods = [0.1, 0.1, 0.1, 0.3]
+    for (n, pose, seed, steps), got in zip(sessions, served):
+        pf = gpu.ParticleFilter()
+        z0, _ = session_draws(seed, 0, n)
+        pf.init(n, *pose, 0.3, 0.3, 0.1, 0.2, z0)
+        for k, (xyz, rg, d, rs) in enumerate(steps):
+            z4, u01 = session_draws(seed, k + 1, n)
+            pf.predict(*mods, *d, z4)
+            pf.update(grid, np.stack(xyz, 1), rg, 0.8, 0.1)
+            if rs:
+                pf.resample(u01)
+        assert np.array_equal(pf.particles().view(np.int32), got.view(np.int32))
+
+
+def test_admission_by_hbm_budget(gpu, room_grid):
+    from amcl3d_amd.serving import Server
+
+    _, grid = room_grid
+    # a session of 10,000 particles holds ~0.8 MB (state, scratch, flags)
+    srv = Server(hbm_budget_bytes=1 << 20)
+    mid = srv.add_map(grid)
+    srv.open_session(mid, 10000, [10, 10, 1.5, 0], [0.1, 0.1, 0.1, 0.1], 1)
+    with pytest.raises(gpu.Amcl3dError) as e:
+        srv.open_session(mid, 10000, [10, 10, 1.5, 0], [0.1, 0.1, 0.1, 0.1], 2)
+    assert e.value.code == 4  # A3D_ERR_NOMEM
+    srv.close()
