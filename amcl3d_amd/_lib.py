@@ -66,6 +66,7 @@ SIGNATURES = {
     "a3d_buffer_free": [_vp],
     "a3d_buffer_upload": [_vp, _vp, _sz],
     "a3d_buffer_download": [_vp, _vp, _sz],
+    "a3d_buffer_copy": [_vp, _vp, _sz],
     "a3d_grid_create_from_points": [_vp, _sz, _vp, _vp, _d, _d, _i, _p(_vp)],
     "a3d_grid_create_from_cells": [_vp, _vp, _vp, _vp, _vp, _d, _d, _p(_vp)],
     "a3d_grid_load": [ctypes.c_char_p, _vp, _vp, _d, _d, _p(_vp)],
@@ -97,6 +98,8 @@ SIGNATURES = {
     "a3d_pf_timing": [_vp, _i, _p(_d), _p(_u64), _p(_u64)],
     "a3d_set_tile": [_i],
     "a3d_get_tile": [_p(_i)],
+    "a3d_set_mapping": [_i],
+    "a3d_get_mapping": [_p(_i)],
     "a3d_fleet_create": [_sz, _vp, _p(_vp)],
     "a3d_fleet_destroy": [_vp],
     "a3d_fleet_set_particles": [_vp, _vp],
@@ -208,6 +211,20 @@ def set_tile(particles_per_wave):
 def get_tile():
     v = _i()
     call("a3d_get_tile", ctypes.byref(v))
+    return v.value
+
+
+MAP_PARTICLE_LANES = 0
+MAP_POINT_LANES = 1
+
+
+def set_mapping(mapping):
+    call("a3d_set_mapping", int(mapping))
+
+
+def get_mapping():
+    v = _i()
+    call("a3d_get_mapping", ctypes.byref(v))
     return v.value
 
 

@@ -17,6 +17,42 @@ import numpy as np
 from . import autotune, fleet as fleet_mod, serving
 
 
+def large_filter_workload(amcl3d_amd, W, steps_total, seed, n, m, nb, log=print):
+    """The single-large-filter workload: a seeded building, 100 x 100 x 30 m at
+    0.05 m (2000 x 2000 x 600 cells) built on the device; a smooth trajectory;
+    per step an m-point cloud (noise 0.03 m), nb beacons, n-particle draws."""
+    t0 = time.time()
+    bm = W.building_map(seed)
+    pts = bm.points(dedup=False)
+    log("map: %d boxes, %d occupied points (%.1fs)" % (bm.boxes.shape[0], pts.shape[0], time.time() - t0))
+    grid = amcl3d_amd.Grid3d()
+    amcl3d_amd.synchronize()
+    t1 = time.time()
+    grid.open(pts, bm.min, bm.max, bm.res, 0.05)
+    amcl3d_amd.synchronize()
+    build_s = time.time() - t1
+    log("computeGrid on the device: %d cells in %.2fs" % (grid.cells, build_s))
+    del pts
+    traj = W.trajectory(bm, steps_total, seed, z=1.5)
+    view = W.storey_view(bm, 1.5, 3.0)
+    rng = np.random.default_rng(seed)
+    anchors = rng.uniform(bm.min + 1, [bm.max[0] - 1, bm.max[1] - 1, 3.0], (nb, 3))
+    clouds, ranges = [], []
+    t2 = time.time()
+    for k in range(steps_total):
+        pose = traj[k + 1]
+        clouds.append(W.sensor_cloud(view, pose, m, 0.03, rng))
+        r = np.linalg.norm(anchors - pose[:3], axis=1) + rng.normal(0, 0.1, nb)
+        ranges.append(np.concatenate([r[:, None], anchors], 1).astype(np.float32))
+    log("clouds: %d x %d points (%.1fs)" % (steps_total, m, time.time() - t2))
+    parts = W.particles_around(traj[0], n, rng)
+    deltas = [W.odom_delta(traj[k], traj[k + 1]) for k in range(steps_total)]
+    z4 = [rng.standard_normal((n, 4)).astype(np.float32) for _ in range(steps_total)]
+    u01 = [float(rng.uniform()) for _ in range(steps_total)]
+    return dict(bm=bm, grid=grid, build_s=build_s, traj=traj, clouds=clouds, ranges=ranges, parts=parts,
+                deltas=deltas, z4=z4, u01=u01, view=view)
+
+
 def room_grid(amcl3d_amd, W, seed=1):
     bm = W.room_map(seed)
     g = amcl3d_amd.Grid3d()
@@ -43,14 +79,14 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3):
     fl = fleet_mod.Fleet(counts)
     fl.set_particles(np.concatenate(parts))
 
-    def prepare(_tile):
+    def prepare(_shape):
         fl.stage_update(clouds, ranges, 0.8, 0.1)
 
     def run_persistent():
         fl.update_staged(grid, fleet_mod.PERSISTENT)
         fl.synchronize()
 
-    tile, timings = autotune.tune(run_persistent, prepare=prepare)
+    shape, timings = autotune.tune(run_persistent, prepare=prepare)
     fl.timing(reset=True)
     t_batched = min(_timed(run_persistent) for _ in range(reps))
     kernel_ms, _, _ = fl.timing(reset=True)
@@ -65,8 +101,8 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3):
     fl.close()
     return {"workload": "%d filters on the 200x200x60 room map, N log-uniform 500..50k, M 1k..30k, 1..8 beacons"
                         % nfilters,
-            "evals_per_update": ev, "tile_particles_per_wave": tile,
-            "autotune_s": {str(k): v for k, v in timings.items()},
+            "evals_per_update": ev, "shape": autotune.shape_name(shape),
+            "autotune_s": {autotune.shape_name(k): v for k, v in timings.items()},
             "batched_evals_per_s": ev / t_batched, "plain_loop_evals_per_s": ev / t_loop,
             "batched_over_plain_loop": t_loop / t_batched, "batched_weight_kernel_ms": kernel_ms,
             "batched_update_ms": t_batched * 1e3, "plain_loop_update_ms": t_loop * 1e3}

@@ -129,7 +129,8 @@ void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, con
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s);
 void launch_predict(const Filters& fs, hipStream_t s);
 void launch_init(const Filters& fs, hipStream_t s);
-void launch_resample(const Filters& fs, int mode, hipStream_t s);
+// have_prefix: the update tail already left the running sum c of these weights.
+void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s);
 // out[4i .. 4i+3] = mean of filter i
 void launch_gather_means(const Filters& fs, float* out, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
@@ -146,7 +147,19 @@ void launch_draws(const Filters& fs, const uint64_t* d_seed, const uint64_t* d_s
 // Cloud-weight kernel tile sizes.
 int clamp_ppw(int ppw);
 int current_ppw();
-size_t scratch_floats(uint32_t n);
+int current_mapping();  // A3D_MAP_PARTICLE_LANES or A3D_MAP_POINT_LANES
+
+// Per-filter scratch layout, in floats (n = particles):
+//   [0, 4n)    update tail: in-map wp, wr, then the weighted x, y, z, a
+//   [4n, 10n)  resample: gathered x, y, z, a, wp, wr
+//   [10n, 12n) resample FP64 prefix (n doubles, 8-byte aligned)
+//   [12n, 13n) resample float running sum c
+//   then FP64 block totals, then 16 floats of per-filter sums (wtp, wtr, wt)
+__host__ __device__ inline size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
+__host__ __device__ inline size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }
+__host__ __device__ inline size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 1) & ~(size_t)1; }
+__host__ __device__ inline size_t sc_sums(uint32_t n) { return sc_tot(n) + 2 * ((size_t)n / 1024 + 2); }
+__host__ __device__ inline size_t scratch_floats(uint32_t n) { return sc_sums(n) + 32; }
 
 // Event pairs around the cloud-weight launches, resolved on demand.
 struct TimingPool {

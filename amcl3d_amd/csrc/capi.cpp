@@ -18,6 +18,7 @@ namespace a3d {
 
 static thread_local std::string g_err;
 static int g_ppw = 32;
+static int g_mapping = A3D_MAP_PARTICLE_LANES;
 
 void set_error(const std::string& msg) { g_err = msg; }
 
@@ -27,10 +28,7 @@ int fail(int code, const std::string& msg) {
 }
 
 int current_ppw() { return g_ppw; }
-
-// Per filter: 12n floats (tail sums, resample running sum, gathered
-// particles, the FP64 prefix) + FP64 block totals.
-size_t scratch_floats(uint32_t n) { return 12 * (size_t)n + 2 * ((size_t)n / 1024 + 2) + 16; }
+int current_mapping() { return g_mapping; }
 
 bool hip_ok(hipError_t e, const char* what) {
   if (e == hipSuccess) return true;
@@ -76,6 +74,7 @@ struct a3d_pf {
   size_t stage_cap = 0;
   bool initialized = false;
   bool capturing = false;
+  bool prefix_valid = false;  // the last update left resample's running sum c
   TimingPool timing;
 
   FilterDev dev() const {
@@ -169,6 +168,12 @@ int a3d_buffer_download(void* host, const void* dptr, size_t bytes) {
   return A3D_OK;
 }
 
+int a3d_buffer_copy(void* dst, const void* src, size_t bytes) {
+  if (!dst || !src) return fail(A3D_ERR_ARG, "a3d_buffer_copy: null pointer");
+  A3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return A3D_OK;
+}
+
 int a3d_set_tile(int particles_per_wave) {
   if (particles_per_wave != 16 && particles_per_wave != 32 && particles_per_wave != 64)
     return fail(A3D_ERR_ARG, "a3d_set_tile: particles per wavefront must be 16, 32 or 64");
@@ -179,6 +184,19 @@ int a3d_set_tile(int particles_per_wave) {
 int a3d_get_tile(int* particles_per_wave) {
   if (!particles_per_wave) return fail(A3D_ERR_ARG, "a3d_get_tile: null output");
   *particles_per_wave = g_ppw;
+  return A3D_OK;
+}
+
+int a3d_set_mapping(int mapping) {
+  if (mapping != A3D_MAP_PARTICLE_LANES && mapping != A3D_MAP_POINT_LANES)
+    return fail(A3D_ERR_ARG, "a3d_set_mapping: unknown mapping");
+  g_mapping = mapping;
+  return A3D_OK;
+}
+
+int a3d_get_mapping(int* mapping) {
+  if (!mapping) return fail(A3D_ERR_ARG, "a3d_get_mapping: null output");
+  *mapping = g_mapping;
   return A3D_OK;
 }
 
@@ -534,6 +552,7 @@ int a3d_pf_init(a3d_pf* p, int num_particles, float x_init, float y_init, float 
   f.pose[3] = a_init;
   f.z4 = p->stage;
   launch_init(single(f), p->stream);
+  p->prefix_valid = false;
   A3D_HIP(hipGetLastError());
   A3D_HIP(hipStreamSynchronize(p->stream));
   p->initialized = true;
@@ -602,6 +621,7 @@ int a3d_pf_update_dev(a3d_pf* p, const a3d_grid* g, const float* d_x, const floa
     p->timing.evals += (uint64_t)p->n * m;
   }
   launch_update_tail(g->g, single(f), p->stream);
+  p->prefix_valid = true;
   A3D_HIP(hipGetLastError());
   return A3D_OK;
 }
@@ -656,7 +676,8 @@ int a3d_pf_resample_dev(a3d_pf* p, float u01, int mode) {
   FilterDev f = p->dev();
   f.u01 = u01;
   f.idx_out = p->idx;
-  launch_resample(single(f), mode, p->stream);
+  launch_resample(single(f), mode, p->prefix_valid, p->stream);
+  p->prefix_valid = false;
   A3D_HIP(hipGetLastError());
   return A3D_OK;
 }
@@ -724,6 +745,7 @@ int a3d_pf_set_particles(a3d_pf* p, const float* p7, size_t n) {
   A3D_HIP(hipMemcpyAsync(p->x, soa.data(), 7 * n * 4, hipMemcpyHostToDevice, p->stream));
   A3D_HIP(hipStreamSynchronize(p->stream));
   p->initialized = true;
+  p->prefix_valid = false;
   return A3D_OK;
 }
 

@@ -1,46 +1,77 @@
 // Particle-filter kernels around the cloud weight:
 //   predict   ParticleFilter.cpp:97-119   one thread per particle
-//   init      ParticleFilter.cpp:46-95    one workgroup per filter
+//   init      ParticleFilter.cpp:46-95
 //   update    ParticleFilter.cpp:150-195  (totals, blend, normalise, mean)
-//   resample  ParticleFilter.cpp:198-222  prefix (one workgroup per filter),
-//                                         then select + gather per particle
+//   resample  ParticleFilter.cpp:198-222  (running sum, select, gather)
 //
 // The reference's sums over particles are sequential float sums in index
-// order (wtp, wtr, wt, the mean, the resample running sum c); the result of
-// resampling depends on every bit of them.  They are computed here as
-// sequential chains too: lane k of wavefront 0 runs chain k while the other
-// wavefronts of the workgroup stage the next block of inputs through LDS.
-// Everything that does not depend on a chain runs in parallel.
+// order (wtp, wtr, wt, the mean, resample's running sum c); resampling
+// depends on every bit of them, so they are computed as sequential chains:
+// chain_kernel runs one workgroup per filter in which lane k of wavefront 0
+// adds chain k while the other fifteen wavefronts stage the next block of
+// inputs through LDS.  Everything that does not depend on a chain runs in
+// grid-wide elementwise kernels (grid y = filter).
+//
+// update:   tail_a  -> chain(wtp, wtr) -> tail_c -> chain(wt) -> tail_e
+//           -> chain(mean x, y, z, a and resample's c)
+// resample: [chain(c) unless update just produced it] -> select -> commit
+// init:     init_a -> chain(wt) -> init_c -> chain(mean)
 #include "a3d_math.h"
 
 namespace a3d {
 namespace {
 
-constexpr int kTailThreads = 1024;
-constexpr int kCH = 1024;  // elements per staged block
+constexpr int kChainThreads = 1024;
+constexpr int kCH = 2048;  // elements per staged block
+constexpr int kEw = 256;   // elementwise block size
 
-// scratch layout (floats), n = particle count
-__device__ __host__ inline size_t sc_off(int k, uint32_t n) { return (size_t)k * n; }
+// which chains a chain_kernel launch runs
+enum ChainWhich {
+  kTailTotals = 0,   // wtp, wtr over the in-map wp, wr     -> sums[0], sums[1]
+  kWeightTotal = 1,  // wt over w                           -> sums[2]
+  kTailMean = 2,     // mean x, y, z, a (+ c over w)        -> mean[0..3], c
+  kPrefixOnly = 3,   // c over w                            -> c
+  kInitMean = 4      // mean x, y, z, a                     -> mean[0..3]
+};
 
-// Sequential float sums.  Lane k < K of wavefront 0 computes
-//   res[k] = (((res[k] + src[k][0]) + src[k][1]) + ...) + src[k][n-1]
-// res[] lives in LDS and is read back by every thread after the call.
-// With PREFIX (K == 1) the running value after each element is stored to
-// prefix[i].
-template <int K, bool PREFIX>
-__device__ void chain_sums(const float* const* src, uint32_t n, float* res, float* stage, float* pstage,
-                           float* prefix) {
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
-  const int nthr = blockDim.x;
-  const uint32_t nblk = (n + kCH - 1) / kCH;
-  float acc = (wave == 0 && lane < K) ? res[lane] : 0.f;
-  if (nblk > 0) {
-    for (int e = tid; e < K * kCH; e += nthr) {
-      const int k = e / kCH, j = e % kCH;
-      stage[k * kCH + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
+__device__ inline const FilterDev& fsel(const Filters& fs, uint32_t i) { return fs.arr ? fs.arr[i] : fs.one; }
+__device__ inline float* sums_of(const FilterDev& f) { return f.scratch + sc_sums(f.n); }
+
+template <int K, int PFX>
+__global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int which) {
+  __shared__ float4 stage4[2 * K * kCH / 4];
+  __shared__ float4 pstage4[PFX >= 0 ? 2 * kCH / 4 : 1];
+  const FilterDev& f = fsel(fs, blockIdx.x);
+  if (which == kPrefixOnly && f.no_resample) return;
+  const uint32_t n = f.n;
+  float* sc = f.scratch;
+  const float* src[K];
+  float* dst[K];
+  float* prefix = nullptr;
+  if (which == kTailTotals) {
+    src[0] = sc;
+    src[1 % K] = sc + n;
+    dst[0] = sums_of(f);
+    dst[1 % K] = sums_of(f) + 1;
+  } else if (which == kWeightTotal || which == kPrefixOnly) {
+    src[0] = f.w;
+    dst[0] = (which == kWeightTotal) ? sums_of(f) + 2 : sums_of(f) + 3;
+  } else {  // kTailMean, kInitMean
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      src[k] = (k < 4) ? sc + (size_t)k * n : f.w;
+      dst[k] = (k < 4) ? f.mean + k : sums_of(f) + 3;
     }
+  }
+  if (PFX >= 0) prefix = sc + sc_c32(n);
+  float* stage = reinterpret_cast<float*>(stage4);
+  float* pst = reinterpret_cast<float*>(pstage4);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, nthr = blockDim.x;
+  const uint32_t nblk = (n + kCH - 1) / kCH;
+  float acc = 0.f;  // every reference sum starts from 0 (0 + x is x exactly)
+  for (int e = tid; e < K * kCH; e += nthr) {
+    const int k = e / kCH, j = e % kCH;
+    stage[k * kCH + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
   }
   __syncthreads();
   for (uint32_t b = 0; b < nblk; ++b) {
@@ -48,50 +79,58 @@ __device__ void chain_sums(const float* const* src, uint32_t n, float* res, floa
     const uint32_t base = b * kCH;
     const uint32_t len = min((uint32_t)kCH, n - base);
     if (wave != 0) {
-      // stage block b+1; write out the prefix of block b-1
       if (b + 1 < nblk) {
         const uint32_t nb = base + kCH;
+#pragma unroll 4
         for (int e = tid - 64; e < K * kCH; e += nthr - 64) {
           const int k = e / kCH, j = e % kCH;
           stage[((buf ^ 1) * K + k) * kCH + j] = (nb + j < n) ? src[k][nb + j] : 0.f;
         }
       }
-      if (PREFIX && b > 0) {
-        const uint32_t pb = base - kCH;
-        for (int j = tid - 64; j < kCH; j += nthr - 64) prefix[pb + j] = pstage[(buf ^ 1) * kCH + j];
-      }
+      if (PFX >= 0 && b > 0)
+        for (int j = tid - 64; j < kCH; j += nthr - 64) prefix[base - kCH + j] = pst[(buf ^ 1) * kCH + j];
     } else if (lane < K) {
-      const float* row = stage + (buf * K + lane) * kCH;
-      float* prow = pstage + buf * kCH;
-      uint32_t j = 0;
-      for (; j + 4 <= len; j += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(row + j);
-        acc += v.x;
-        if (PREFIX) prow[j] = acc;
-        acc += v.y;
-        if (PREFIX) prow[j + 1] = acc;
-        acc += v.z;
-        if (PREFIX) prow[j + 2] = acc;
-        acc += v.w;
-        if (PREFIX) prow[j + 3] = acc;
+      const float4* row4 = reinterpret_cast<const float4*>(stage + (buf * K + lane) * kCH);
+      float4* prow4 = reinterpret_cast<float4*>(pst + buf * kCH);
+      const uint32_t n4 = len >> 2;
+      uint32_t q = 0;
+      for (; q + 16 <= n4; q += 16) {
+        float4 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = row4[q + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          float4 o;
+          acc += v[u].x;
+          o.x = acc;
+          acc += v[u].y;
+          o.y = acc;
+          acc += v[u].z;
+          o.z = acc;
+          acc += v[u].w;
+          o.w = acc;
+          if (PFX >= 0 && lane == PFX) prow4[q + u] = o;
+        }
       }
-      for (; j < len; ++j) {
+      const float* row = reinterpret_cast<const float*>(row4);
+      for (uint32_t j = q * 4; j < len; ++j) {
         acc += row[j];
-        if (PREFIX) prow[j] = acc;
+        if (PFX >= 0 && lane == PFX) pst[buf * kCH + j] = acc;
       }
     }
     __syncthreads();
   }
-  if (PREFIX && nblk > 0) {
+  if (PFX >= 0 && nblk > 0) {
     const uint32_t lb = (nblk - 1) * kCH;
     const int buf = (nblk - 1) & 1;
-    for (uint32_t j = tid; lb + j < n && j < (uint32_t)kCH; j += nthr) prefix[lb + j] = pstage[buf * kCH + j];
+    for (uint32_t j = tid; j < (uint32_t)kCH && lb + j < n; j += nthr) prefix[lb + j] = pst[buf * kCH + j];
   }
-  if (wave == 0 && lane < K) res[lane] = acc;
-  __syncthreads();
+  if (wave == 0 && lane < K) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (lane == k) *dst[k] = acc;
+  }
 }
-
-__device__ inline const FilterDev& fsel(const Filters& fs, uint32_t i) { return fs.arr ? fs.arr[i] : fs.one; }
 
 // ----------------------------------------------------------------- predict
 __global__ void predict_kernel(Filters fs) {
@@ -111,139 +150,104 @@ __global__ void predict_kernel(Filters fs) {
 }
 
 // -------------------------------------------------------------------- init
-// gauss constants of ParticleFilter.cpp:54-56, computed on the host:
-// f.dev[0] = gauss_const_1, f.dev[1] = gauss_const_2 (as floats in doubles),
-// f.pose = initial pose, f.alpha/sigma unused; x/y/z/a deviations in
-// f.delta[0..3].
-__global__ __launch_bounds__(kTailThreads) void init_kernel(Filters fs) {
-  __shared__ float4 stage4[2 * 4 * kCH / 4];
-  __shared__ float res[8];
-  float* stage = reinterpret_cast<float*>(stage4);
-  const FilterDev& f = fsel(fs, blockIdx.x);
-  const uint32_t n = f.n;
+// gauss constants of ParticleFilter.cpp:54-56 come from the host in f.dev[0]
+// (gauss_const_1) and f.dev[1] (gauss_const_2); deviations in f.delta[0..3].
+__global__ void init_a_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
   const float gc1 = (float)f.dev[0];
   const float gc2 = (float)f.dev[1];
   const float x0 = f.pose[0], y0 = f.pose[1], z0 = f.pose[2], a0 = f.pose[3];
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    if (i == 0) {
-      f.x[0] = x0;
-      f.y[0] = y0;
-      f.z[0] = z0;
-      f.a[0] = a0;
-      f.w[0] = gc1;
-      continue;
-    }
-    const float* z = f.z4 + 4 * (size_t)i;
-    const float xi = x0 + gauss_draw(z[0], (double)(float)f.delta[0]);
-    const float yi = y0 + gauss_draw(z[1], (double)(float)f.delta[1]);
-    const float zi = z0 + gauss_draw(z[2], (double)(float)f.delta[2]);
-    const float ai = a0 + gauss_draw(z[3], (double)(float)f.delta[3]);
-    f.x[i] = xi;
-    f.y[i] = yi;
-    f.z[i] = zi;
-    f.a[i] = ai;
-    const float dist = sqrtf((xi - x0) * (xi - x0) + (yi - y0) * (yi - y0) + (zi - z0) * (zi - z0));
-    f.w[i] = (float)((double)gc1 * exp((double)(-dist * dist * gc2)));
+  if (i == 0) {
+    f.x[0] = x0;
+    f.y[0] = y0;
+    f.z[0] = z0;
+    f.a[0] = a0;
+    f.w[0] = gc1;
+    return;
   }
-  if (threadIdx.x == 0) res[0] = 0.f;
-  __syncthreads();
-  const float* s1[1] = {f.w};
-  chain_sums<1, false>(s1, n, res, stage, nullptr, nullptr);
-  const float wt = res[0];
+  const float* z = f.z4 + 4 * (size_t)i;
+  const float xi = x0 + gauss_draw(z[0], (double)(float)f.delta[0]);
+  const float yi = y0 + gauss_draw(z[1], (double)(float)f.delta[1]);
+  const float zi = z0 + gauss_draw(z[2], (double)(float)f.delta[2]);
+  const float ai = a0 + gauss_draw(z[3], (double)(float)f.delta[3]);
+  f.x[i] = xi;
+  f.y[i] = yi;
+  f.z[i] = zi;
+  f.a[i] = ai;
+  const float dist = sqrtf((xi - x0) * (xi - x0) + (yi - y0) * (yi - y0) + (zi - z0) * (zi - z0));
+  f.w[i] = (float)((double)gc1 * exp((double)(-dist * dist * gc2)));
+}
+
+// w /= wt (float division, ParticleFilter.cpp:85) and the weighted terms of the mean
+__global__ void init_c_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  const uint32_t n = f.n;
+  const float wt = sums_of(f)[2];
+  const float w = f.w[i] / wt;
+  f.w[i] = w;
   float* sc = f.scratch;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const float w = f.w[i] / wt;
-    f.w[i] = w;
-    sc[sc_off(0, n) + i] = w * f.x[i];
-    sc[sc_off(1, n) + i] = w * f.y[i];
-    sc[sc_off(2, n) + i] = w * f.z[i];
-    sc[sc_off(3, n) + i] = w * f.a[i];
-  }
-  if (threadIdx.x < 4) res[threadIdx.x] = 0.f;
-  __syncthreads();
-  const float* s4[4] = {sc + sc_off(0, n), sc + sc_off(1, n), sc + sc_off(2, n), sc + sc_off(3, n)};
-  chain_sums<4, false>(s4, n, res, stage, nullptr, nullptr);
-  if (threadIdx.x < 4) f.mean[threadIdx.x] = res[threadIdx.x];
+  sc[i] = w * f.x[i];
+  sc[n + i] = w * f.y[i];
+  sc[2 * (size_t)n + i] = w * f.z[i];
+  sc[3 * (size_t)n + i] = w * f.a[i];
 }
 
 // ------------------------------------------------------------- update tail
-__global__ __launch_bounds__(kTailThreads) void tail_kernel(GridDev g, Filters fs) {
-  __shared__ float4 stage4[2 * 4 * kCH / 4];
-  __shared__ float res[8];
-  float* stage = reinterpret_cast<float*>(stage4);
-  const FilterDev& f = fsel(fs, blockIdx.x);
+// ParticleFilter.cpp:129-153: out-of-map particles get w = 0 and add nothing
+// to wtp, wtr (adding +0.0f to a non-negative sum is exact).
+__global__ void tail_a_kernel(GridDev g, Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  const bool in = into_map(g, f.x[i], f.y[i], f.z[i]);
+  f.inmap[i] = in ? 1 : 0;
+  f.scratch[i] = in ? f.wp[i] : 0.f;
+  f.scratch[f.n + i] = in ? f.wr[i] : 0.f;
+}
+
+// ParticleFilter.cpp:158-180
+__global__ void tail_c_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  const float* s = sums_of(f);
+  const float wtp = s[0], wtr = s[1];
+  const float wp = (wtp > 0) ? f.wp[i] / wtp : 0.f;
+  const float wr = (wtr > 0) ? f.wr[i] / wtr : 0.f;
+  f.wp[i] = wp;
+  f.wr[i] = wr;
+  f.w[i] = f.inmap[i] ? (float)(wp * f.alpha + wr * (1 - f.alpha)) : 0.f;
+}
+
+// ParticleFilter.cpp:182-193: w /= wt and the weighted terms of the mean
+__global__ void tail_e_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
   const uint32_t n = f.n;
+  const float wt = sums_of(f)[2];
+  const float w = (wt > 0) ? f.w[i] / wt : 0.f;
+  f.w[i] = w;
   float* sc = f.scratch;
-  // ParticleFilter.cpp:129-153: out-of-map particles get w = 0 and add
-  // nothing to wtp, wtr (adding +0.0f to a non-negative sum is exact).
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const bool in = into_map(g, f.x[i], f.y[i], f.z[i]);
-    f.inmap[i] = in ? 1 : 0;
-    sc[sc_off(0, n) + i] = in ? f.wp[i] : 0.f;
-    sc[sc_off(1, n) + i] = in ? f.wr[i] : 0.f;
-  }
-  if (threadIdx.x < 2) res[threadIdx.x] = 0.f;
-  __syncthreads();
-  {
-    const float* s2[2] = {sc + sc_off(0, n), sc + sc_off(1, n)};
-    chain_sums<2, false>(s2, n, res, stage, nullptr, nullptr);
-  }
-  const float wtp = res[0], wtr = res[1];
-  // ParticleFilter.cpp:158-180
-  const double alpha = f.alpha;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const float wp = (wtp > 0) ? f.wp[i] / wtp : 0.f;
-    const float wr = (wtr > 0) ? f.wr[i] / wtr : 0.f;
-    f.wp[i] = wp;
-    f.wr[i] = wr;
-    f.w[i] = f.inmap[i] ? (float)(wp * alpha + wr * (1 - alpha)) : 0.f;
-  }
-  if (threadIdx.x == 0) res[0] = 0.f;
-  __syncthreads();
-  {
-    const float* s1[1] = {f.w};
-    chain_sums<1, false>(s1, n, res, stage, nullptr, nullptr);
-  }
-  const float wt = res[0];
-  // ParticleFilter.cpp:182-195
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const float w = (wt > 0) ? f.w[i] / wt : 0.f;
-    f.w[i] = w;
-    sc[sc_off(0, n) + i] = w * f.x[i];
-    sc[sc_off(1, n) + i] = w * f.y[i];
-    sc[sc_off(2, n) + i] = w * f.z[i];
-    sc[sc_off(3, n) + i] = w * f.a[i];
-  }
-  if (threadIdx.x < 4) res[threadIdx.x] = 0.f;
-  __syncthreads();
-  {
-    const float* s4[4] = {sc + sc_off(0, n), sc + sc_off(1, n), sc + sc_off(2, n), sc + sc_off(3, n)};
-    chain_sums<4, false>(s4, n, res, stage, nullptr, nullptr);
-  }
-  if (threadIdx.x < 4) f.mean[threadIdx.x] = res[threadIdx.x];
+  sc[i] = w * f.x[i];
+  sc[n + i] = w * f.y[i];
+  sc[2 * (size_t)n + i] = w * f.z[i];
+  sc[3 * (size_t)n + i] = w * f.a[i];
 }
 
 // ---------------------------------------------------------------- resample
-// Scratch use: [0, n) float running sum c (mode 0); [4n, 10n) gathered
-// x, y, z, a, wp, wr; [10n, 12n) double prefix (mode 1); then block totals.
-__global__ __launch_bounds__(kTailThreads) void resample_prefix_kernel(Filters fs, int mode) {
-  __shared__ float4 stage4[2 * kCH / 4];
-  __shared__ float4 pstage4[2 * kCH / 4];
-  __shared__ float res[8];
+// FP64 prefix: sequential double sums in blocks of 1024, block totals summed
+// in order (one workgroup per filter).
+__global__ __launch_bounds__(1024) void prefix64_kernel(Filters fs) {
   const FilterDev& f = fsel(fs, blockIdx.x);
   if (f.no_resample) return;
   const uint32_t n = f.n;
-  float* sc = f.scratch;
-  if (mode == A3D_RESAMPLE_REFERENCE) {
-    // c = p_[0].w; c += p_[i].w ...: the running values c_i (ParticleFilter.cpp:203,214)
-    if (threadIdx.x == 0) res[0] = 0.f;
-    __syncthreads();
-    const float* s1[1] = {f.w};
-    chain_sums<1, true>(s1, n, res, reinterpret_cast<float*>(stage4), reinterpret_cast<float*>(pstage4), sc);
-    return;
-  }
-  double* c64 = reinterpret_cast<double*>(sc + sc_off(10, n));
-  double* tot = c64 + n;
+  double* c64 = reinterpret_cast<double*>(f.scratch + sc_c64(n));
+  double* tot = reinterpret_cast<double*>(f.scratch + sc_tot(n));
   const uint32_t nb = (n + 1023) / 1024;
   for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
     double s = 0.0;
@@ -295,19 +299,19 @@ __global__ void resample_select_kernel(Filters fs, int mode) {
     const float factor = 1.f / (float)n;  // 1.f / p_.size() (size_t -> float)
     const float r = factor * u01;
     const float u = r + factor * (float)m;
-    i = first_not_below(sc, n, u);
+    i = first_not_below(sc + sc_c32(n), n, u);
   } else {
-    const double* c64 = reinterpret_cast<const double*>(sc + sc_off(10, n));
+    const double* c64 = reinterpret_cast<const double*>(sc + sc_c64(n));
     const double u = ((double)u01 + (double)m) * (1.0 / (double)n);
     i = first_not_below(c64, n, u);
   }
   if (f.idx_out) f.idx_out[m] = i;
-  sc[sc_off(4, n) + m] = f.x[i];
-  sc[sc_off(5, n) + m] = f.y[i];
-  sc[sc_off(6, n) + m] = f.z[i];
-  sc[sc_off(7, n) + m] = f.a[i];
-  sc[sc_off(8, n) + m] = f.wp[i];
-  sc[sc_off(9, n) + m] = f.wr[i];
+  sc[4 * (size_t)n + m] = f.x[i];
+  sc[5 * (size_t)n + m] = f.y[i];
+  sc[6 * (size_t)n + m] = f.z[i];
+  sc[7 * (size_t)n + m] = f.a[i];
+  sc[8 * (size_t)n + m] = f.wp[i];
+  sc[9 * (size_t)n + m] = f.wr[i];
 }
 
 __global__ void resample_commit_kernel(Filters fs) {
@@ -316,12 +320,12 @@ __global__ void resample_commit_kernel(Filters fs) {
   const uint32_t n = f.n;
   if (m >= n || f.no_resample) return;
   const float* sc = f.scratch;
-  f.x[m] = sc[sc_off(4, n) + m];
-  f.y[m] = sc[sc_off(5, n) + m];
-  f.z[m] = sc[sc_off(6, n) + m];
-  f.a[m] = sc[sc_off(7, n) + m];
-  f.wp[m] = sc[sc_off(8, n) + m];
-  f.wr[m] = sc[sc_off(9, n) + m];
+  f.x[m] = sc[4 * (size_t)n + m];
+  f.y[m] = sc[5 * (size_t)n + m];
+  f.z[m] = sc[6 * (size_t)n + m];
+  f.a[m] = sc[7 * (size_t)n + m];
+  f.wp[m] = sc[8 * (size_t)n + m];
+  f.wr[m] = sc[9 * (size_t)n + m];
   f.w[m] = 1.f / (float)n;
 }
 
@@ -332,6 +336,8 @@ __global__ void gather_means_kernel(Filters fs, float* out) {
   for (int k = 0; k < 4; ++k) out[4 * (size_t)i + k] = f.mean[k];
 }
 
+inline dim3 ew_grid(const Filters& fs) { return dim3((fs.max_n + kEw - 1) / kEw, fs.count); }
+
 }  // namespace
 
 void launch_gather_means(const Filters& fs, float* out, hipStream_t s) {
@@ -341,26 +347,37 @@ void launch_gather_means(const Filters& fs, float* out, hipStream_t s) {
 
 void launch_predict(const Filters& fs, hipStream_t s) {
   if (fs.max_n == 0) return;
-  const dim3 grid((fs.max_n + 255) / 256, fs.count);
-  predict_kernel<<<grid, 256, 0, s>>>(fs);
+  predict_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
 }
 
 void launch_init(const Filters& fs, hipStream_t s) {
-  if (fs.count == 0) return;
-  init_kernel<<<fs.count, kTailThreads, 0, s>>>(fs);
+  if (fs.count == 0 || fs.max_n == 0) return;
+  init_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+  chain_kernel<1, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
+  init_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+  chain_kernel<4, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kInitMean);
 }
 
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
-  if (fs.count == 0) return;
-  tail_kernel<<<fs.count, kTailThreads, 0, s>>>(g, fs);
+  if (fs.count == 0 || fs.max_n == 0) return;
+  tail_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(g, fs);
+  chain_kernel<2, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kTailTotals);
+  tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+  chain_kernel<1, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
+  tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+  // the mean and, in lane 4, resample's running sum c over the new weights
+  chain_kernel<5, 4><<<fs.count, kChainThreads, 0, s>>>(fs, kTailMean);
 }
 
-void launch_resample(const Filters& fs, int mode, hipStream_t s) {
+void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
-  resample_prefix_kernel<<<fs.count, kTailThreads, 0, s>>>(fs, mode);
-  const dim3 grid((fs.max_n + 255) / 256, fs.count);
-  resample_select_kernel<<<grid, 256, 0, s>>>(fs, mode);
-  resample_commit_kernel<<<grid, 256, 0, s>>>(fs);
+  if (mode == A3D_RESAMPLE_REFERENCE) {
+    if (!have_prefix) chain_kernel<1, 0><<<fs.count, kChainThreads, 0, s>>>(fs, kPrefixOnly);
+  } else {
+    prefix64_kernel<<<fs.count, 1024, 0, s>>>(fs);
+  }
+  resample_select_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs, mode);
+  resample_commit_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
 }
 
 }  // namespace a3d

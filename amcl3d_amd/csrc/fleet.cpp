@@ -41,8 +41,10 @@ struct a3d_fleet {
   WorkItem* d_items = nullptr;
   size_t items_cap = 0;
   uint32_t nitems = 0;
+  int items_ppw = 32;  // particles per work item of the staged list
   uint32_t* d_counter = nullptr;
   bool staged = false;
+  bool prefix_valid = false;  // the last update left resample's running sum c
   std::vector<uint32_t> st_m, st_nb;
   std::vector<double> st_alpha, st_sigma, st_roll, st_pitch;
   std::vector<uint64_t> st_moff, st_boff;
@@ -162,6 +164,7 @@ int a3d_fleet_set_particles(a3d_fleet* fl, const float* p7) {
     for (int k = 0; k < 7; ++k) soa[k * fl->total + i] = p7[7 * i + k];
   A3D_HIP(hipMemcpyAsync(fl->state, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, fl->stream));
   A3D_HIP(hipStreamSynchronize(fl->stream));
+  fl->prefix_valid = false;
   return A3D_OK;
 }
 
@@ -196,6 +199,7 @@ int a3d_fleet_init(a3d_fleet* fl, const float* poses, const float* devs, const f
   rc = upload_filters(fl);
   if (rc) return rc;
   launch_init(fl->device_filters(), fl->stream);
+  fl->prefix_valid = false;
   A3D_HIP(hipGetLastError());
   A3D_HIP(hipStreamSynchronize(fl->stream));
   return A3D_OK;
@@ -288,6 +292,7 @@ int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const 
                          fl->stream));
   A3D_HIP(hipStreamSynchronize(fl->stream));  // host vectors go out of scope
   fl->nitems = (uint32_t)items.size();
+  fl->items_ppw = ppw;
   fl->st_m.assign(m, m + fl->F);
   fl->staged = true;
   return A3D_OK;
@@ -299,7 +304,8 @@ int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent) {
   auto ev = fl->timing.take();
   if (ev.first) (void)hipEventRecord(ev.first, fl->stream);
   if (persistent >= 0) {
-    launch_weight(g->g, fl->device_filters(), fl->d_items, fl->nitems, fl->d_counter, current_ppw(), persistent,
+    // the list was cut for the tile current at staging time; launch with that tile
+    launch_weight(g->g, fl->device_filters(), fl->d_items, fl->nitems, fl->d_counter, fl->items_ppw, persistent,
                   fl->stream);
   } else {
     // the plain single-filter kernel, launched once per filter
@@ -319,6 +325,7 @@ int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent) {
   } else {
     for (uint32_t f = 0; f < fl->F; ++f) launch_update_tail(g->g, single(fl->h_filters[f]), fl->stream);
   }
+  fl->prefix_valid = true;
   A3D_HIP(hipGetLastError());
   return A3D_OK;
 }
@@ -347,7 +354,8 @@ int a3d_fleet_resample(a3d_fleet* fl, const float* u01, int mode) {
   }
   int rc = upload_filters(fl);
   if (rc) return rc;
-  launch_resample(fl->device_filters(), mode, fl->stream);
+  launch_resample(fl->device_filters(), mode, fl->prefix_valid, fl->stream);
+  fl->prefix_valid = false;
   A3D_HIP(hipGetLastError());
   A3D_HIP(hipStreamSynchronize(fl->stream));
   return A3D_OK;

@@ -294,7 +294,7 @@ bool a3d_server::launch(Slot& s) {
   if (any_rs) {
     Filters fr = fs;
     fr.u01 = s.d_u01;
-    launch_resample(fr, A3D_RESAMPLE_REFERENCE, st);
+    launch_resample(fr, A3D_RESAMPLE_REFERENCE, true, st);
   }
   (void)hipEventRecord(s.done, st);
   if (hipGetLastError() != hipSuccess) {

@@ -23,8 +23,10 @@ constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16
 constexpr int kWaves = 4;      // wavefronts per workgroup
 constexpr int kThreads = 64 * kWaves;
 
+// Cell of one point under one pose: returns whether it is in the map and
+// its index (0 when not, so that the gather is always in bounds).
 template <bool RP0, bool IDX64>
-__device__ __forceinline__ float gather_one(const GridDev& g, float px, float py, float pz, bool pv, const float* r,
+__device__ __forceinline__ uint64_t cell_of(const GridDev& g, float px, float py, float pz, bool pv, const float* r,
                                             double ox, double oy, double oz, bool& valid) {
   float sx, sy, sz;
   if (RP0) {
@@ -39,21 +41,15 @@ __device__ __forceinline__ float gather_one(const GridDev& g, float px, float py
     sy = px * r[3] + py * r[4] + pz * r[5];
     sz = px * r[6] + py * r[7] + pz * r[8];
   }
-  uint32_t ix, iy, iz;
-  valid = pv && axis_cell(sx, ox, g.ext_hi[0], g.res, g.inv_res, g.size[0], ix) &&
-          axis_cell(sy, oy, g.ext_hi[1], g.res, g.inv_res, g.size[1], iy) &&
-          axis_cell(sz, oz, g.ext_hi[2], g.res, g.inv_res, g.size[2], iz);
-  float v = 0.f;
-  if (valid) {
-    if (IDX64) {
-      const uint64_t idx = (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
-      v = g.prob[idx];
-    } else {
-      const uint32_t idx = ix + iy * g.size[0] + iz * (uint32_t)g.step_z;
-      v = g.prob[idx];
-    }
-  }
-  return v;
+  uint32_t ix = 0, iy = 0, iz = 0;
+  bool ok = pv;
+  ok = axis_cell(sx, ox, g.ext_hi[0], g.res, g.inv_res, g.size[0], ix) && ok;
+  ok = axis_cell(sy, oy, g.ext_hi[1], g.res, g.inv_res, g.size[1], iy) && ok;
+  ok = axis_cell(sz, oz, g.ext_hi[2], g.res, g.inv_res, g.size[2], iz) && ok;
+  valid = ok;
+  if (!ok) return 0;
+  if (IDX64) return (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
+  return (uint64_t)(ix + iy * g.size[0] + iz * (uint32_t)g.step_z);
 }
 
 // Evaluates the 64 points held by the lanes for every particle of the block
@@ -62,25 +58,43 @@ template <bool RP0, bool IDX64>
 __device__ __forceinline__ void eval_chunk(const GridDev& g, float px, float py, float pz, bool pv,
                                            const float (&r)[9], double ox, double oy, double oz, uint32_t count,
                                            uint64_t active, float* tile, int& cnt) {
+  // Four particles per iteration: their cells are computed first and the
+  // four gathers issued together, so each wavefront keeps four loads in
+  // flight.  Rows of particles outside the map are not written (never read).
+  constexpr int U = 4;
   const int lane = lane_id();
-  for (uint32_t i = 0; i < count; ++i) {
-    if (!((active >> i) & 1ull)) continue;
-    float ri[9];
-    if (RP0) {
-      ri[0] = readlane_f(r[0], i);
-      ri[1] = readlane_f(r[1], i);
-      ri[3] = readlane_f(r[3], i);
-      ri[4] = readlane_f(r[4], i);
-    } else {
+  for (uint32_t i0 = 0; i0 < count; i0 += U) {
+    uint64_t idx[U];
+    bool val[U];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) ri[k] = readlane_f(r[k], i);
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u;  // <= 63: count <= 64 and i0 is a multiple of U
+      const bool act = i < count && ((active >> i) & 1ull);
+      float ri[9];
+      if (RP0) {
+        ri[0] = readlane_f(r[0], i);
+        ri[1] = readlane_f(r[1], i);
+        ri[3] = readlane_f(r[3], i);
+        ri[4] = readlane_f(r[4], i);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ri[k] = readlane_f(r[k], i);
+      }
+      const double oxi = readlane_d(ox, i), oyi = readlane_d(oy, i), ozi = readlane_d(oz, i);
+      idx[u] = cell_of<RP0, IDX64>(g, px, py, pz, pv && act, ri, oxi, oyi, ozi, val[u]);
     }
-    const double oxi = readlane_d(ox, i), oyi = readlane_d(oy, i), ozi = readlane_d(oz, i);
-    bool valid;
-    const float v = gather_one<RP0, IDX64>(g, px, py, pz, pv, ri, oxi, oyi, ozi, valid);
-    tile[i * kTS + lane] = v;
-    const uint64_t vm = __ballot(valid);
-    cnt += (lane == (int)i) ? (int)__popcll(vm) : 0;
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u;
+      if (i < count && ((active >> i) & 1ull)) {
+        tile[i * kTS + lane] = val[u] ? v[u] : 0.f;
+        const uint64_t vm = __ballot(val[u]);
+        cnt += (lane == (int)i) ? (int)__popcll(vm) : 0;
+      }
+    }
   }
 }
 
@@ -154,6 +168,174 @@ __device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mapping 0, lanes = particles.  A wavefront carries P = 64/Q particles; lane
+// l holds particle l % P and, at each step, point (step*Q + l / P) of the
+// cloud.  A particle's sum takes the Q values of a step from its Q lanes in
+// point order (lane shuffles), so it is the reference's sequential float sum.
+// Particles that resampling made near-duplicates sit in neighbouring lanes:
+// one gather then touches few cache lines.
+template <bool RP0, bool IDX64>
+__device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float py, float pz, const float (&r)[9],
+                                              double ox, double oy, double oz, bool act, bool& valid) {
+  float sx, sy, sz;
+  if (RP0) {
+    sx = px * r[0] + py * r[1];
+    sy = px * r[3] + py * r[4];
+    sz = pz;
+  } else {
+    sx = px * r[0] + py * r[1] + pz * r[2];
+    sy = px * r[3] + py * r[4] + pz * r[5];
+    sz = px * r[6] + py * r[7] + pz * r[8];
+  }
+  bool okx, oky, okz, nx, ny, nz;
+  uint32_t ix, iy, iz;
+  float qx, qy, qz;
+  axis_fast(sx, ox, g.ext_hi[0], g.inv_res, g.size[0], okx, nx, ix, qx);
+  axis_fast(sy, oy, g.ext_hi[1], g.inv_res, g.size[1], oky, ny, iy, qy);
+  axis_fast(sz, oz, g.ext_hi[2], g.inv_res, g.size[2], okz, nz, iz, qz);
+  if (__builtin_expect(nx | ny | nz, 0)) {
+    if (nx) axis_exact(qx, g.ext_hi[0], g.res, g.size[0], okx, ix);
+    if (ny) axis_exact(qy, g.ext_hi[1], g.res, g.size[1], oky, iy);
+    if (nz) axis_exact(qz, g.ext_hi[2], g.res, g.size[2], okz, iz);
+  }
+  valid = act & okx & oky & okz;
+  if (IDX64) return valid ? (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z : 0ull;
+  return valid ? (uint64_t)(ix + iy * g.size[0] + iz * (uint32_t)g.step_z) : 0ull;
+}
+
+template <int Q, bool RP0, bool IDX64>
+__device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const float (&r)[9], double ox,
+                                           double oy, double oz, bool act, float& wsum, int& cnt) {
+  constexpr int P = 64 / Q;
+  constexpr int U = (Q == 1) ? 8 : 4;  // steps in flight
+  const int lane = lane_id();
+  const int ps = lane % P, po = lane / P;
+  const uint32_t m = f.m;
+  const float error_z = 0.f;
+  auto accumulate = [&](float v, bool val) {
+    const float vv = val ? v : 0.0f;  // +0.0f leaves the non-negative sum unchanged
+    if (Q == 1) {
+      wsum += vv;
+      cnt += val ? 1 : 0;
+    } else {
+      const uint64_t bm = __ballot(val);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float vq = (q == 0) ? vv : __shfl(vv, ps + q * P);
+        wsum += vq;
+        cnt += (int)((bm >> (ps + q * P)) & 1ull);
+      }
+    }
+  };
+  uint32_t j0 = 0;
+  for (; j0 + U * Q <= m; j0 += U * Q) {
+    uint64_t idx[U];
+    bool val[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + u * Q + po;
+      const float px = f.cx[j], py = f.cy[j], pz = f.cz[j] + error_z;
+      const bool fin = !RP0 || (isfinite(px) && isfinite(py) && isfinite(pz));
+      idx[u] = lane_cell<RP0, IDX64>(g, px, py, pz, r, ox, oy, oz, act & fin, val[u]);
+    }
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) accumulate(v[u], val[u]);
+  }
+  for (; j0 < m; j0 += Q) {
+    const uint32_t j = j0 + po;
+    const bool pin = j < m;
+    const uint32_t js = pin ? j : 0u;
+    const float px = f.cx[js], py = f.cy[js], pz = f.cz[js] + error_z;
+    const bool fin = !RP0 || (isfinite(px) && isfinite(py) && isfinite(pz));
+    bool val;
+    const uint64_t idx = lane_cell<RP0, IDX64>(g, px, py, pz, r, ox, oy, oz, act & fin & pin, val);
+    accumulate(g.prob[idx], val);
+  }
+}
+
+template <int Q, bool IDX64, bool KFILTER>
+__device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out) {
+  constexpr int P = 64 / Q;
+  const int lane = lane_id();
+  const int ps = lane % P, po = lane / P;
+  const bool has = (uint32_t)ps < count;
+  const uint32_t pi = p0 + (has ? (uint32_t)ps : 0u);
+  float tx = 0.f, ty = 0.f, tz = 0.f, ta = 0.f;
+  if (has) {
+    tx = f.x[pi];
+    ty = f.y[pi];
+    tz = f.z[pi];
+    ta = f.a[pi];
+  }
+  const bool act = KFILTER ? (has && into_map(g, tx, ty, tz)) : has;
+  if (__ballot(act) == 0) return;
+  float r[9];
+  rotation9(f.roll, f.pitch, ta, r);
+  const double ox = (double)tx - g.min[0];
+  const double oy = (double)ty - g.min[1];
+  const double oz = (double)tz - g.min[2];
+  float wr = 0.f;
+  if (KFILTER && act && po == 0) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
+  float wsum = 0.f;
+  int cnt = 0;
+  if (f.rp0)
+    lanes_loop<Q, true, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt);
+  else
+    lanes_loop<Q, false, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt);
+  if (act && po == 0) {
+    const float wp = (cnt <= 10) ? 0.f : wsum / (float)cnt;
+    if (KFILTER) {
+      f.wp[pi] = wp;
+      f.wr[pi] = wr;
+    } else {
+      out[pi] = wp;
+    }
+  }
+}
+
+template <int Q, bool IDX64>
+__global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
+  constexpr uint32_t P = 64 / Q;
+  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t p0 = wave * P;
+  if (p0 >= f.n) return;
+  weight_lanes<Q, IDX64, true>(g, f, p0, min(P, f.n - p0), nullptr);
+}
+
+template <int Q, bool IDX64>
+__global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
+  constexpr uint32_t P = 64 / Q;
+  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t p0 = wave * P;
+  if (p0 >= f.n) return;
+  weight_lanes<Q, IDX64, false>(g, f, p0, min(P, f.n - p0), out);
+}
+
+template <int Q, bool IDX64>
+__global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs, const WorkItem* items,
+                                                               uint32_t nitems, uint32_t* counter, int persistent) {
+  const int lane = lane_id();
+  uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (!persistent) {
+    if (k >= nitems) return;
+    const WorkItem it = items[k];
+    weight_lanes<Q, IDX64, true>(g, fs[it.filter], it.p0, it.count, nullptr);
+    return;
+  }
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= nitems) break;
+    const WorkItem it = items[t];
+    weight_lanes<Q, IDX64, true>(g, fs[it.filter], it.p0, it.count, nullptr);
+  }
+}
+
 template <int PB, bool IDX64>
 __global__ __launch_bounds__(kThreads) void weight_single_kernel(GridDev g, FilterDev f) {
   __shared__ float4 tiles4[kWaves * PB * kTS / 4];
@@ -192,7 +374,7 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
   if (!persistent) {
     if (k >= nitems) return;
     const WorkItem it = items[k];
-    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, it.count, tile, nullptr);
+    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, min(it.count, (uint32_t)PB), tile, nullptr);
     return;
   }
   for (;;) {
@@ -201,7 +383,7 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
     t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
     if (t >= nitems) break;
     const WorkItem it = items[t];
-    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, it.count, tile, nullptr);
+    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, min(it.count, (uint32_t)PB), tile, nullptr);
   }
 }
 
@@ -258,6 +440,57 @@ void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* item
     weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
 }
 
+template <int Q>
+void lanes_single_q(const GridDev& g, const FilterDev& f, hipStream_t s) {
+  constexpr uint32_t P = 64 / Q;
+  const uint32_t waves = (f.n + P - 1) / P;
+  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  if (g.idx64)
+    lanes_single_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, f);
+  else
+    lanes_single_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, f);
+}
+
+template <int Q>
+void lanes_poses_q(const GridDev& g, const FilterDev& f, float* out, hipStream_t s) {
+  constexpr uint32_t P = 64 / Q;
+  const uint32_t waves = (f.n + P - 1) / P;
+  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  if (g.idx64)
+    lanes_poses_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, f, out);
+  else
+    lanes_poses_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, f, out);
+}
+
+template <int Q>
+void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items, uint32_t nitems, uint32_t* counter,
+                   int persistent, hipStream_t s) {
+  uint32_t blocks;
+  if (persistent) {
+    static int cus = 0, per_cu[2] = {0, 0};
+    if (!cus) {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, true>, kThreads, 0);
+      if (cus < 1) cus = 256;
+    }
+    blocks = (uint32_t)(cus * std::max(1, per_cu[g.idx64 ? 1 : 0]));
+    const uint32_t need = (nitems + kWaves - 1) / kWaves;
+    if (blocks > need) blocks = need;
+    if (blocks == 0) blocks = 1;
+    zero_u32_kernel<<<1, 64, 0, s>>>(counter, 1);
+  } else {
+    blocks = (nitems + kWaves - 1) / kWaves;
+  }
+  if (blocks == 0) return;
+  if (g.idx64)
+    lanes_fleet_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else
+    lanes_fleet_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+}
+
 }  // namespace
 
 int clamp_ppw(int ppw) {
@@ -269,6 +502,20 @@ int clamp_ppw(int ppw) {
 void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
                    uint32_t* d_counter, int ppw, int persistent, hipStream_t s) {
   ppw = clamp_ppw(ppw);
+  if (current_mapping() == 0) {  // lanes = particles, ppw particles per wavefront
+    if (!fs.arr) {
+      if (fs.one.n == 0) return;
+      if (ppw == 64) lanes_single_q<1>(g, fs.one, s);
+      else if (ppw == 32) lanes_single_q<2>(g, fs.one, s);
+      else lanes_single_q<4>(g, fs.one, s);
+      return;
+    }
+    if (nitems == 0) return;
+    if (ppw == 64) lanes_fleet_q<1>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+    else if (ppw == 32) lanes_fleet_q<2>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+    else lanes_fleet_q<4>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+    return;
+  }
   if (!fs.arr) {
     if (fs.one.n == 0) return;
     if (ppw == 16) launch_single_pb<16>(g, fs.one, s);
@@ -300,6 +547,12 @@ void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, con
   f.pitch = pitch;
   f.rp0 = (roll == 0.0f && pitch == 0.0f) ? 1u : 0u;
   ppw = clamp_ppw(ppw);
+  if (current_mapping() == 0) {
+    if (ppw == 64) lanes_poses_q<1>(g, f, out, s);
+    else if (ppw == 32) lanes_poses_q<2>(g, f, out, s);
+    else lanes_poses_q<4>(g, f, out, s);
+    return;
+  }
   if (ppw == 16) launch_poses_pb<16>(g, f, out, s);
   else if (ppw == 32) launch_poses_pb<32>(g, f, out, s);
   else launch_poses_pb<64>(g, f, out, s);

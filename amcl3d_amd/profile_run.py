@@ -1,0 +1,61 @@
+"""Profiling driver: the cloud-weight kernel of the large-filter workload,
+then a device copy of known size.
+
+Under ``rocprofv3 --pmc ...`` the copy kernel's FETCH_SIZE / WRITE_SIZE
+(it reads and writes ``--copy-mib`` MiB) calibrate the counters that the
+cloud-weight dispatches report.
+
+    python -m amcl3d_amd.profile_run --updates 3 --copy-mib 1024
+"""
+import argparse
+import sys
+import time
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--updates", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--particles", type=int, default=100000)
+    ap.add_argument("--points", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--tile", type=int, default=0, help="particles per wavefront (0: library default)")
+    args = ap.parse_args(argv)
+
+    import amcl3d_amd
+    from amcl3d_amd import _lib, benchlib, workloads as W
+
+    amcl3d_amd.load()
+    if args.tile:
+        amcl3d_amd.set_tile(args.tile)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    wl = benchlib.large_filter_workload(amcl3d_amd, W, args.updates, args.seed, args.particles, args.points, 8,
+                                        log=log)
+    pf = amcl3d_amd.ParticleFilter()
+    pf.set_particles(wl["parts"])
+    mods = np.array([0.1, 0.1, 0.1, 0.3])
+    for k in range(args.updates):
+        c = wl["clouds"][k]
+        x, y, z = (amcl3d_amd.DeviceBuffer.from_array(np.ascontiguousarray(c[:, j])) for j in range(3))
+        r = amcl3d_amd.DeviceBuffer.from_array(wl["ranges"][k])
+        dz = amcl3d_amd.DeviceBuffer.from_array(wl["z4"][k])
+        pf.predict_dev(mods, wl["deltas"][k], dz)
+        pf.update_dev(wl["grid"], x.ptr, y.ptr, z.ptr, args.points, r.ptr, 8, 0.8, 0.1)
+        pf.resample_dev(wl["u01"][k])
+        pf.synchronize()
+    ms, n, ev = pf.timing()
+    log("cloud weight: %d launches, %.3f ms average, %.3e evals/s" % (n, ms, ev / n / (ms * 1e-3)))
+    nbytes = args.copy_mib << 20
+    a = amcl3d_amd.DeviceBuffer(nbytes)
+    b = amcl3d_amd.DeviceBuffer(nbytes)
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _lib.call("a3d_buffer_copy", b.ptr, a.ptr, nbytes)
+        log("copy %d MiB: %.3f ms" % (args.copy_mib, (time.perf_counter() - t0) * 1e3))
+    print("profile_run done", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -39,36 +39,9 @@ def log(*a):
 
 
 def build_workload(amcl3d_amd, W, steps_total, seed, n, m, nb):
-    t0 = time.time()
-    bm = W.building_map(seed)
-    pts = bm.points(dedup=False)
-    log("map: %d boxes, %d occupied points (%.1fs)" % (bm.boxes.shape[0], pts.shape[0], time.time() - t0))
-    grid = amcl3d_amd.Grid3d()
-    amcl3d_amd.synchronize()
-    t1 = time.time()
-    grid.open(pts, bm.min, bm.max, bm.res, 0.05)
-    amcl3d_amd.synchronize()
-    build_s = time.time() - t1
-    log("computeGrid on the device: %d cells in %.2fs" % (grid.cells, build_s))
-    del pts
-    traj = W.trajectory(bm, steps_total, seed, z=1.5)
-    view = W.storey_view(bm, 1.5, 3.0)
-    rng = np.random.default_rng(seed)
-    anchors = rng.uniform(bm.min + 1, [bm.max[0] - 1, bm.max[1] - 1, 3.0], (nb, 3))
-    clouds, ranges = [], []
-    t2 = time.time()
-    for k in range(steps_total):
-        pose = traj[k + 1]
-        clouds.append(W.sensor_cloud(view, pose, m, 0.03, rng))
-        r = np.linalg.norm(anchors - pose[:3], axis=1) + rng.normal(0, 0.1, nb)
-        ranges.append(np.concatenate([r[:, None], anchors], 1).astype(np.float32))
-    log("clouds: %d x %d points (%.1fs)" % (steps_total, m, time.time() - t2))
-    parts = W.particles_around(traj[0], n, rng)
-    deltas = [W.odom_delta(traj[k], traj[k + 1]) for k in range(steps_total)]
-    z4 = [rng.standard_normal((n, 4)).astype(np.float32) for _ in range(steps_total)]
-    u01 = [float(rng.uniform()) for _ in range(steps_total)]
-    return dict(bm=bm, grid=grid, build_s=build_s, traj=traj, clouds=clouds, ranges=ranges, parts=parts,
-                deltas=deltas, z4=z4, u01=u01)
+    from amcl3d_amd import benchlib
+
+    return benchlib.large_filter_workload(amcl3d_amd, W, steps_total, seed, n, m, nb, log=log)
 
 
 def main():
@@ -83,6 +56,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--shape", default="auto",
+                    help="cloud-weight kernel shape 'MAPPING,PPW' (e.g. 0,32) or 'auto' to time every shape")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the filter path has no multi-GPU form "
@@ -124,6 +99,8 @@ def main():
         enqueue(k)
     pf.synchronize()
     start_state = pf.particles()
+    tuning = choose_shape(amcl3d_amd, pf, grid, dbuf[Wm], args, alpha, sigma)
+    pf.set_particles(start_state)
     pf.capture_begin()
     for k in range(Wm, Wm + K):
         enqueue(k)
@@ -183,11 +160,12 @@ def main():
             "beacons": args.beacons,
             "cells": int(grid.cells),
             "seed": args.seed,
-            "tile_particles_per_wave": amcl3d_amd.get_tile(),
+            "kernel_shape": tuning,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "cloud weight (weight_single_kernel)",
+            "kernel": "cloud weight (%s)" % ("lanes_single_kernel" if amcl3d_amd.get_mapping() == 0
+                                             else "weight_single_kernel"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -213,6 +191,27 @@ def main():
         except Exception as e:  # extras never hide the main line
             out["extras"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
+
+
+def choose_shape(amcl3d_amd, pf, grid, bufs, args, alpha, sigma):
+    """Kernel shape for the timed run: the one given, or the fastest of every
+    shape on the update of the first timed step (all shapes give identical bits)."""
+    from amcl3d_amd import autotune
+
+    if args.shape != "auto":
+        sh = tuple(int(v) for v in args.shape.split(","))
+        autotune.set_shape(sh)
+        return {"chosen": autotune.shape_name(sh)}
+    x, y, z, r, _ = bufs
+
+    def run():
+        pf.update_dev(grid, x.ptr, y.ptr, z.ptr, args.points, r.ptr, args.beacons, alpha, sigma)
+        pf.synchronize()
+
+    best, timings = autotune.tune(run)
+    log("shape autotune:", {autotune.shape_name(k): round(v * 1e3, 3) for k, v in timings.items()}, "ms")
+    return {"chosen": autotune.shape_name(best),
+            "update_ms": {autotune.shape_name(k): v * 1e3 for k, v in timings.items()}}
 
 
 def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):

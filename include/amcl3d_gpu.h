@@ -77,6 +77,9 @@ int a3d_buffer_alloc(size_t bytes, void** dptr);
 int a3d_buffer_free(void* dptr);
 int a3d_buffer_upload(void* dptr, const void* host, size_t bytes);
 int a3d_buffer_download(void* host, const void* dptr, size_t bytes);
+/* Device-to-device copy (also the reference copy kernel for HBM-byte
+ * counter calibration). */
+int a3d_buffer_copy(void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------- Grid3d */
 /* Grid3d::open minus the octomap/.grid handling (Grid3d.cpp:25-78): builds the
@@ -153,10 +156,17 @@ int a3d_graph_destroy(a3d_graph* graph);
  * Not available inside a capture. */
 int a3d_pf_timing(a3d_pf* pf, int reset, double* weight_ms_avg, uint64_t* launches, uint64_t* evals);
 
-/* Tile shape of the cloud-weight kernel: particles per wavefront (8..64, a
- * multiple of 8).  Any shape gives identical results. */
+/* Shape of the cloud-weight kernel; every shape gives identical results.
+ * Mapping A3D_MAP_PARTICLE_LANES: a wavefront carries `particles_per_wave`
+ * particles (64, 32 or 16), each lane one particle and 64/particles_per_wave
+ * consecutive points per step.  A3D_MAP_POINT_LANES: lanes are 64
+ * consecutive points, a wavefront walks the cloud for `particles_per_wave`
+ * particles through an LDS tile. */
+enum { A3D_MAP_PARTICLE_LANES = 0, A3D_MAP_POINT_LANES = 1 };
 int a3d_set_tile(int particles_per_wave);
 int a3d_get_tile(int* particles_per_wave);
+int a3d_set_mapping(int mapping);
+int a3d_get_mapping(int* mapping);
 
 /* -------------------------------------------------------------------- Fleet */
 /* Many independent filters on one grid, updated by one persistent launch.

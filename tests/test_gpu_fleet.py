@@ -54,22 +54,32 @@ def test_fleet_matches_single_filters(gpu):
         args = [_lib.ptr(x), _lib.ptr(y), _lib.ptr(z), _lib.u32(ms), _lib.f32(R),
                 _lib.u32(nbs), _lib.f64(alpha), _lib.f64(sigma),
                 _lib.f64(roll), _lib.f64(pitch)]
-        for persistent in (1, 0, -1):
-            _lib.call("a3d_fleet_set_particles", fl, _lib.f32(P))
-            _lib.call("a3d_fleet_stage_update", fl, *args)
-            _lib.call("a3d_fleet_update_staged", fl, grid.handle, persistent)
-            _lib.call("a3d_fleet_synchronize", fl)
-            out = np.empty_like(P)
-            _lib.call("a3d_fleet_get_particles", fl, _lib.ptr(out))
-            means = np.empty((F, 4), F32)
-            _lib.call("a3d_fleet_get_means", fl, _lib.ptr(means))
-            off = 0
-            for f in range(F):
-                op, om = oracle.pf_update(parts[f], og, clouds[f], rngs[f], alpha[f], sigma[f], roll[f], pitch[f])
-                n = int(counts[f])
-                assert np.array_equal(out[off:off + n].view(np.int32), op.view(np.int32)), (persistent, f)
-                assert np.array_equal(means[f].view(np.int32), om.view(np.int32)), (persistent, f)
-                off += n
+        expect = [oracle.pf_update(parts[f], og, clouds[f], rngs[f], alpha[f], sigma[f], roll[f], pitch[f])
+                  for f in range(F)]
+        m0 = gpu.get_mapping()
+        try:
+            for mapping in (gpu.MAP_PARTICLE_LANES, gpu.MAP_POINT_LANES):
+                gpu.set_mapping(mapping)
+                for persistent in (1, 0, -1):
+                    print("fleet mapping", mapping, "persistent", persistent, flush=True)
+                    _lib.call("a3d_fleet_set_particles", fl, _lib.f32(P))
+                    _lib.call("a3d_fleet_stage_update", fl, *args)
+                    _lib.call("a3d_fleet_update_staged", fl, grid.handle, persistent)
+                    _lib.call("a3d_fleet_synchronize", fl)
+                    out = np.empty_like(P)
+                    _lib.call("a3d_fleet_get_particles", fl, _lib.ptr(out))
+                    means = np.empty((F, 4), F32)
+                    _lib.call("a3d_fleet_get_means", fl, _lib.ptr(means))
+                    off = 0
+                    for f in range(F):
+                        op, om = expect[f]
+                        n = int(counts[f])
+                        key = (mapping, persistent, f)
+                        assert np.array_equal(out[off:off + n].view(np.int32), op.view(np.int32)), key
+                        assert np.array_equal(means[f].view(np.int32), om.view(np.int32)), key
+                        off += n
+        finally:
+            gpu.set_mapping(m0)
         _lib.call("a3d_fleet_resample", fl, _lib.ptr(u01), 0)
         res = np.empty_like(P)
         _lib.call("a3d_fleet_get_particles", fl, _lib.ptr(res))

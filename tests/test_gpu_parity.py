@@ -95,19 +95,32 @@ def test_grid_empty_and_tiny(gpu):
 
 
 # --------------------------------------------------------- computeCloudWeight
-@pytest.mark.parametrize("tile", [16, 32, 64])
-def test_cloud_weights_bit_exact(gpu, room, tile):
-    gpu.set_tile(tile)
-    try:
-        s, p = room["step"], room["particles"]
-        w = room["grid"].cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
-        ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
-        assert np.array_equal(w.view(np.int32), ow.view(np.int32))
-    finally:
-        gpu.set_tile(32)
+@pytest.fixture(params=[(0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)],
+                ids=lambda mt: "map%d-ppw%d" % mt)
+def shape(gpu, request):
+    """Every kernel shape (mapping, particles per wavefront) must give the same bits."""
+    m0, t0 = gpu.get_mapping(), gpu.get_tile()
+    gpu.set_mapping(request.param[0])
+    gpu.set_tile(request.param[1])
+    _say("cloud-weight shape mapping=%d ppw=%d" % request.param)
+    yield request.param
+    gpu.set_mapping(m0)
+    gpu.set_tile(t0)
 
 
-def test_cloud_weights_roll_pitch_and_edges(gpu, room):
+def test_cloud_weights_bit_exact(gpu, room, shape):
+    s, p = room["step"], room["particles"]
+    w = room["grid"].cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+    # ragged cloud sizes: the unrolled main loop plus every tail length
+    for m in (11, 12, 37, 64 * 8 + 5):
+        w = room["grid"].cloud_weights(s.cloud[:m], p[:200, 0], p[:200, 1], p[:200, 2], p[:200, 3])
+        ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud[:m], p[:200, 0], p[:200, 1], p[:200, 2], p[:200, 3])
+        assert np.array_equal(w.view(np.int32), ow.view(np.int32)), m
+
+
+def test_cloud_weights_roll_pitch_and_edges(gpu, room, shape):
     rng = np.random.default_rng(11)
     s = room["step"]
     n = 300
@@ -142,7 +155,7 @@ def test_cloud_weights_roll_pitch_and_edges(gpu, room):
     assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
-def test_cloud_weights_64bit_index_path(gpu, room, monkeypatch):
+def test_cloud_weights_64bit_index_path(gpu, room, shape, monkeypatch):
     monkeypatch.setenv("A3D_FORCE_IDX64", "1")
     bm = room["map"]
     g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
