@@ -62,6 +62,7 @@ struct FilterDev {
   float* scratch;  // scratch_floats(n): tail sums, resample prefix, gather staging
   uint8_t* inmap;  // n
   float* mean;     // 4
+  const GridDev* gdev;  // device copy of this filter's grid; null: the launch's grid
   uint32_t n;
   // update
   uint32_t m;

@@ -17,7 +17,7 @@
 namespace a3d {
 
 static thread_local std::string g_err;
-static int g_ppw = 32;
+static int g_ppw = 16;
 static int g_mapping = A3D_MAP_PARTICLE_LANES;
 
 void set_error(const std::string& msg) { g_err = msg; }

@@ -203,7 +203,7 @@ __global__ void tail_a_kernel(GridDev g, Filters fs) {
   const FilterDev& f = fsel(fs, blockIdx.y);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.n) return;
-  const bool in = into_map(g, f.x[i], f.y[i], f.z[i]);
+  const bool in = into_map(f.gdev ? *f.gdev : g, f.x[i], f.y[i], f.z[i]);
   f.inmap[i] = in ? 1 : 0;
   f.scratch[i] = in ? f.wp[i] : 0.f;
   f.scratch[f.n + i] = in ? f.wr[i] : 0.f;

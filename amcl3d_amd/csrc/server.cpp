@@ -141,6 +141,7 @@ struct a3d_server {
   a3d_server_config cfg{};
   int device = 0;
   std::vector<const a3d_grid*> maps;
+  std::vector<GridDev*> d_maps;  // device copy of each map's view (mixed-map batches)
   std::vector<Session> sessions;
   size_t hbm_used = 0;
   std::mutex mu;
@@ -250,6 +251,13 @@ bool a3d_server::launch(Slot& s) {
     s.evals += (uint64_t)se.n * r.m;
   }
   for (size_t i = 0; i < nf; ++i) s.h_seedstep.push_back(sessions[s.reqs[i].session].step);
+  // a batch over several maps: every filter names its grid, and one launch
+  // serves them all
+  const int map0 = sessions[s.reqs[0].session].map;
+  bool mixed = false;
+  for (auto& r : s.reqs) mixed |= sessions[r.session].map != map0;
+  if (mixed)
+    for (size_t i = 0; i < nf; ++i) s.h_filters[i].gdev = d_maps[sessions[s.reqs[i].session].map];
   if (!grow_dev(s.d_items, s.items_cap, s.h_items.size())) return false;
   hipStream_t st = s.stream;
   bool ok = hipMemcpyAsync(s.d_data, s.h_data, off * 4, hipMemcpyHostToDevice, st) == hipSuccess &&
@@ -268,23 +276,8 @@ bool a3d_server::launch(Slot& s) {
   fs.max_n = max_n;
   launch_draws(fs, s.d_seedstep, s.d_seedstep + nf, s.d_u01, st);
   launch_predict(fs, st);
-  // the grids of one batch may differ: one cloud-weight launch per map
-  std::vector<int> used;
-  for (auto& r : s.reqs)
-    if (std::find(used.begin(), used.end(), sessions[r.session].map) == used.end())
-      used.push_back(sessions[r.session].map);
-  if (used.size() == 1) {
-    launch_weight(maps[used[0]]->g, fs, s.d_items, (uint32_t)s.h_items.size(), s.d_counter, ppw, 1, st);
-    launch_update_tail(maps[used[0]]->g, fs, st);
-  } else {
-    for (int mp : used) {
-      for (size_t i = 0; i < nf; ++i) {
-        if (sessions[s.reqs[i].session].map != mp) continue;
-        launch_weight(maps[mp]->g, single(s.h_filters[i]), nullptr, 0, nullptr, ppw, 0, st);
-        launch_update_tail(maps[mp]->g, single(s.h_filters[i]), st);
-      }
-    }
-  }
+  launch_weight(maps[map0]->g, fs, s.d_items, (uint32_t)s.h_items.size(), s.d_counter, ppw, 1, st);
+  launch_update_tail(maps[map0]->g, fs, st);
   // means to the host before resample (getMean is read after update, Node.cpp:167)
   launch_gather_means(fs, s.d_means, st);
   (void)hipMemcpyAsync(s.h_mean, s.d_means, 16 * nf, hipMemcpyDeviceToHost, st);
@@ -440,6 +433,7 @@ int a3d_server_destroy(a3d_server* S) {
   }
   for (auto& se : S->sessions)
     if (se.open) free_session(S, se);
+  for (GridDev* d : S->d_maps) (void)hipFree(d);
   if (S->admin) (void)hipStreamDestroy(S->admin);
   delete S;
   return A3D_OK;
@@ -448,7 +442,14 @@ int a3d_server_destroy(a3d_server* S) {
 int a3d_server_add_map(a3d_server* S, const a3d_grid* g, int* map_id) {
   if (!S || !g || !map_id) return fail(A3D_ERR_ARG, "a3d_server_add_map: null argument");
   std::lock_guard<std::mutex> lk(S->mu);
+  GridDev* d = nullptr;
+  if (hipMalloc(&d, sizeof(GridDev)) != hipSuccess) return fail(A3D_ERR_NOMEM, "a3d_server_add_map: allocation");
+  if (hipMemcpy(d, &g->g, sizeof(GridDev), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return fail(A3D_ERR_HIP, "a3d_server_add_map: copy");
+  }
   S->maps.push_back(g);
+  S->d_maps.push_back(d);
   *map_id = (int)S->maps.size() - 1;
   return A3D_OK;
 }

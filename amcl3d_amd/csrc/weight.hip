@@ -315,6 +315,19 @@ __global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, Filter
   weight_lanes<Q, IDX64, false>(g, f, p0, min(P, f.n - p0), out);
 }
 
+// One work item; a filter that names its own grid (a batch over several
+// maps) is weighted on that grid, else on the launch's.
+template <int Q, bool IDX64>
+__device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it) {
+  const uint32_t count = min(it.count, (uint32_t)(64 / Q));
+  if (f.gdev) {
+    const GridDev gf = *f.gdev;
+    weight_lanes<Q, IDX64, true>(gf, f, it.p0, count, nullptr);
+  } else {
+    weight_lanes<Q, IDX64, true>(g, f, it.p0, count, nullptr);
+  }
+}
+
 template <int Q, bool IDX64>
 __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs, const WorkItem* items,
                                                                uint32_t nitems, uint32_t* counter, int persistent) {
@@ -323,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
   if (!persistent) {
     if (k >= nitems) return;
     const WorkItem it = items[k];
-    weight_lanes<Q, IDX64, true>(g, fs[it.filter], it.p0, it.count, nullptr);
+    lanes_item<Q, IDX64>(g, fs[it.filter], it);
     return;
   }
   for (;;) {
@@ -331,8 +344,7 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
     if (lane == 0) t = atomicAdd(counter, 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
     if (t >= nitems) break;
-    const WorkItem it = items[t];
-    weight_lanes<Q, IDX64, true>(g, fs[it.filter], it.p0, it.count, nullptr);
+    lanes_item<Q, IDX64>(g, fs[items[t].filter], items[t]);
   }
 }
 
@@ -358,6 +370,17 @@ __global__ __launch_bounds__(kThreads) void weight_poses_kernel(GridDev g, Filte
   weight_block<PB, IDX64, false>(g, f, p0, count, tile, out);
 }
 
+template <int PB, bool IDX64>
+__device__ __forceinline__ void block_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float* tile) {
+  const uint32_t count = min(it.count, (uint32_t)PB);
+  if (f.gdev) {
+    const GridDev gf = *f.gdev;
+    weight_block<PB, IDX64, true>(gf, f, it.p0, count, tile, nullptr);
+  } else {
+    weight_block<PB, IDX64, true>(g, f, it.p0, count, tile, nullptr);
+  }
+}
+
 // Many filters: work items (filter, first particle, count) are taken either
 // one per wavefront (grid sized to the list) or, when persistent, from an
 // atomic counter by resident wavefronts until the list is exhausted.  Items
@@ -373,8 +396,7 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
   uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (!persistent) {
     if (k >= nitems) return;
-    const WorkItem it = items[k];
-    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, min(it.count, (uint32_t)PB), tile, nullptr);
+    block_item<PB, IDX64>(g, fs[items[k].filter], items[k], tile);
     return;
   }
   for (;;) {
@@ -382,8 +404,7 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
     if (lane == 0) t = atomicAdd(counter, 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
     if (t >= nitems) break;
-    const WorkItem it = items[t];
-    weight_block<PB, IDX64, true>(g, fs[it.filter], it.p0, min(it.count, (uint32_t)PB), tile, nullptr);
+    block_item<PB, IDX64>(g, fs[items[t].filter], items[t], tile);
   }
 }
 

@@ -3,6 +3,7 @@
 No compute call is made here; the GPU tests call through the same ABI.
 """
 import ctypes
+import os
 import re
 
 import numpy as np
@@ -83,3 +84,14 @@ def test_product_path_has_no_cpu_fallback():
             _lib.load()
         finally:
             _lib.LIB_PATH, _lib._LIB = saved
+
+
+def test_cpp_layer_soft_failures_without_a_map():
+    """Grid3d without an open map: computeCloudWeight 0 and isIntoMap false
+    (Grid3d.cpp:136-137, 203-204), through the C++ class layer."""
+    import subprocess
+
+    shim = os.path.join(os.path.dirname(_lib.LIB_PATH), "shim_check")
+    r = subprocess.run([shim, "unopened"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert '"weight": 0' in r.stdout and '"into_map": 0' in r.stdout

@@ -33,12 +33,16 @@ def _sessions(bm, k, seed):
     return out
 
 
-def _run(gpu, grid, sessions, alone):
+def _run(gpu, grid, sessions, alone, maps_of=None):
+    """grid: one grid, or a list of grids with maps_of[i] the map of session i."""
     from amcl3d_amd.serving import Server
 
     srv = Server(max_inflight=1 if alone else 3, max_batch_sessions=1 if alone else 64)
-    mid = srv.add_map(grid)
-    ids = [srv.open_session(mid, n, pose, [0.3, 0.3, 0.1, 0.2], seed) for n, pose, seed, _ in sessions]
+    grids = grid if isinstance(grid, list) else [grid]
+    mids = [srv.add_map(g) for g in grids]
+    maps_of = maps_of or [0] * len(sessions)
+    ids = [srv.open_session(mids[maps_of[i]], n, pose, [0.3, 0.3, 0.1, 0.2], seed)
+           for i, (n, pose, seed, _) in enumerate(sessions)]
     mods = [0.1, 0.1, 0.1, 0.3]
     for k in range(3):
         for sid, (_, _, _, steps) in zip(ids, sessions):
@@ -86,6 +90,40 @@ def test_served_equals_plain_filter(gpu, room_grid):
             if rs:
                 pf.resample(u01)
         assert np.array_equal(pf.particles().view(np.int32), got.view(np.int32))
+
+
+def test_mixed_maps_in_one_batch(gpu, room_grid):
+    """Sessions on two different maps share batches (one launch, each filter
+    on its own grid) and still give their alone results and the plain
+    filter's."""
+    from amcl3d_amd.serving import session_draws
+
+    bm, grid = room_grid
+    bm2 = W.room_map(2, extent=(12.0, 16.0, 5.0), res=0.08)
+    g2 = gpu.Grid3d()
+    g2.open(bm2.points(), bm2.min, bm2.max, bm2.res, 0.15)
+    sessions = _sessions(bm, 4, 21) + _sessions(bm2, 4, 22)
+    maps_of = [0] * 4 + [1] * 4
+    order = [0, 4, 1, 5, 2, 6, 3, 7]  # interleave the maps
+    sessions = [sessions[i] for i in order]
+    maps_of = [maps_of[i] for i in order]
+    alone, _, _ = _run(gpu, [grid, g2], sessions, alone=True, maps_of=maps_of)
+    mixed, _, st = _run(gpu, [grid, g2], sessions, alone=False, maps_of=maps_of)
+    assert st["batches"] < 24
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # a session on the second map against the plain filter
+    n, pose, seed, steps = sessions[1]
+    pf = gpu.ParticleFilter()
+    z0, _ = session_draws(seed, 0, n)
+    pf.init(n, *pose, 0.3, 0.3, 0.1, 0.2, z0)
+    for k, (xyz, rg, d, rs) in enumerate(steps):
+        z4, u01 = session_draws(seed, k + 1, n)
+        pf.predict(0.1, 0.1, 0.1, 0.3, *d, z4)
+        pf.update(g2, np.stack(xyz, 1), rg, 0.8, 0.1)
+        if rs:
+            pf.resample(u01)
+    assert np.array_equal(pf.particles().view(np.int32), mixed[1].view(np.int32))
 
 
 def test_admission_by_hbm_budget(gpu, room_grid):
