@@ -39,12 +39,13 @@ class Amcl3dError(RuntimeError):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_evals", ctypes.c_uint64),
-                ("max_batch_sessions", ctypes.c_int), ("max_inflight", ctypes.c_int)]
+                ("max_batch_sessions", ctypes.c_int), ("max_inflight", ctypes.c_int), ("latest_only", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("ticket", ctypes.c_uint64), ("session", ctypes.c_int), ("mean", ctypes.c_float * 4),
-                ("latency_ms", ctypes.c_double), ("submit_ms", ctypes.c_double), ("evals", ctypes.c_uint64)]
+                ("latency_ms", ctypes.c_double), ("submit_ms", ctypes.c_double), ("evals", ctypes.c_uint64),
+                ("deltas", ctypes.c_double * 4), ("superseded", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -100,6 +101,8 @@ SIGNATURES = {
     "a3d_get_tile": [_p(_i)],
     "a3d_set_mapping": [_i],
     "a3d_get_mapping": [_p(_i)],
+    "a3d_set_order": [_i],
+    "a3d_get_order": [_p(_i)],
     "a3d_fleet_create": [_sz, _vp, _p(_vp)],
     "a3d_fleet_destroy": [_vp],
     "a3d_fleet_set_particles": [_vp, _vp],
@@ -117,6 +120,7 @@ SIGNATURES = {
     "a3d_server_destroy": [_vp],
     "a3d_server_add_map": [_vp, _vp, _p(_i)],
     "a3d_server_open_session": [_vp, _i, _i, _vp, _vp, _u64, _p(_i)],
+    "a3d_server_open_session_particles": [_vp, _i, _vp, _sz, _u64, _p(_i)],
     "a3d_server_close_session": [_vp, _i],
     "a3d_server_submit": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _d, _d, _d, _d, _i, _p(_u64)],
     "a3d_server_poll": [_vp, _p(Result), _sz, _p(_sz)],
@@ -225,6 +229,20 @@ def set_mapping(mapping):
 def get_mapping():
     v = _i()
     call("a3d_get_mapping", ctypes.byref(v))
+    return v.value
+
+
+ORDER_INDEX = 0
+ORDER_SPATIAL = 1
+
+
+def set_order(order):
+    call("a3d_set_order", int(order))
+
+
+def get_order():
+    v = _i()
+    call("a3d_get_order", ctypes.byref(v))
     return v.value
 
 

@@ -63,6 +63,7 @@ struct FilterDev {
   uint8_t* inmap;  // n
   float* mean;     // 4
   const GridDev* gdev;  // device copy of this filter's grid; null: the launch's grid
+  const uint32_t* order;  // cloud-weight visiting order of the particles; null: index order
   uint32_t n;
   // update
   uint32_t m;
@@ -113,6 +114,24 @@ inline Filters single(const FilterDev& f) {
   return s;
 }
 
+// Buffers of the particle visiting order (launch_order).
+constexpr uint32_t kOrderKeys = 65536;
+struct OrderDev {
+  uint32_t* key;    // n
+  uint32_t* order;  // n
+  uint32_t* hist;   // kOrderKeys + 1
+  float* box;       // 8: min x, y, z, a; then 16 / extent of each
+};
+inline size_t order_bytes(uint32_t n) { return (2 * (size_t)n + kOrderKeys + 1) * 4 + 64; }
+inline OrderDev order_view(void* base, uint32_t n) {
+  OrderDev o;
+  o.key = static_cast<uint32_t*>(base);
+  o.order = o.key + n;
+  o.hist = o.order + n;
+  o.box = reinterpret_cast<float*>(o.hist + kOrderKeys + 1);
+  return o;
+}
+
 // Host-side grid metadata, computed exactly as the reference does.
 void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g);
 
@@ -135,6 +154,8 @@ void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t 
 // out[4i .. 4i+3] = mean of filter i
 void launch_gather_means(const Filters& fs, float* out, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+// Visiting order of f's particles for the cloud weight, into o.order.
+void launch_order(const FilterDev& f, const OrderDev& o, hipStream_t s);
 
 // Device map build (computeGrid, PointCloudTools.cpp:84-149).
 int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_prob, float* d_dist,
@@ -149,6 +170,7 @@ void launch_draws(const Filters& fs, const uint64_t* d_seed, const uint64_t* d_s
 int clamp_ppw(int ppw);
 int current_ppw();
 int current_mapping();  // A3D_MAP_PARTICLE_LANES or A3D_MAP_POINT_LANES
+int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 
 // Per-filter scratch layout, in floats (n = particles):
 //   [0, 4n)    update tail: in-map wp, wr, then the weighted x, y, z, a

@@ -19,6 +19,7 @@ namespace a3d {
 static thread_local std::string g_err;
 static int g_ppw = 16;
 static int g_mapping = A3D_MAP_PARTICLE_LANES;
+static int g_order = A3D_ORDER_SPATIAL;
 
 void set_error(const std::string& msg) { g_err = msg; }
 
@@ -29,6 +30,7 @@ int fail(int code, const std::string& msg) {
 
 int current_ppw() { return g_ppw; }
 int current_mapping() { return g_mapping; }
+int current_order() { return g_order; }
 
 bool hip_ok(hipError_t e, const char* what) {
   if (e == hipSuccess) return true;
@@ -70,6 +72,7 @@ struct a3d_pf {
   uint8_t* inmap = nullptr;
   float* mean = nullptr;
   uint32_t* idx = nullptr;
+  void* ord = nullptr;     // particle visiting order buffers (order_view)
   float* stage = nullptr;  // host-input staging
   size_t stage_cap = 0;
   bool initialized = false;
@@ -191,6 +194,18 @@ int a3d_set_mapping(int mapping) {
   if (mapping != A3D_MAP_PARTICLE_LANES && mapping != A3D_MAP_POINT_LANES)
     return fail(A3D_ERR_ARG, "a3d_set_mapping: unknown mapping");
   g_mapping = mapping;
+  return A3D_OK;
+}
+
+int a3d_set_order(int order) {
+  if (order != A3D_ORDER_INDEX && order != A3D_ORDER_SPATIAL) return fail(A3D_ERR_ARG, "a3d_set_order: unknown order");
+  g_order = order;
+  return A3D_OK;
+}
+
+int a3d_get_order(int* order) {
+  if (!order) return fail(A3D_ERR_ARG, "a3d_get_order: null output");
+  *order = g_order;
   return A3D_OK;
 }
 
@@ -468,9 +483,11 @@ static void pf_free_state(a3d_pf* p) {
   (void)hipFree(p->inmap);
   (void)hipFree(p->mean);
   (void)hipFree(p->idx);
+  (void)hipFree(p->ord);
   p->x = p->scratch = p->mean = nullptr;
   p->inmap = nullptr;
   p->idx = nullptr;
+  p->ord = nullptr;
   p->n = 0;
 }
 
@@ -494,14 +511,16 @@ static int pf_resize(a3d_pf* p, uint32_t n) {
   uint8_t* in = nullptr;
   float* mean = nullptr;
   uint32_t* idx = nullptr;
+  void* ord = nullptr;
   if (hipMalloc(&x, 7 * (size_t)n * 4) != hipSuccess || hipMalloc(&sc, scratch_floats(n) * 4) != hipSuccess ||
       hipMalloc(&in, n) != hipSuccess || hipMalloc(&mean, 4 * 4) != hipSuccess ||
-      hipMalloc(&idx, (size_t)n * 4) != hipSuccess) {
+      hipMalloc(&idx, (size_t)n * 4) != hipSuccess || hipMalloc(&ord, order_bytes(n)) != hipSuccess) {
     (void)hipFree(x);
     (void)hipFree(sc);
     (void)hipFree(in);
     (void)hipFree(mean);
     (void)hipFree(idx);
+    (void)hipFree(ord);
     return fail(A3D_ERR_NOMEM, "particle filter: cannot allocate particle state");
   }
   A3D_HIP(hipMemsetAsync(x, 0, 7 * (size_t)n * 4, p->stream));
@@ -517,6 +536,7 @@ static int pf_resize(a3d_pf* p, uint32_t n) {
   p->inmap = in;
   p->mean = mean;
   p->idx = idx;
+  p->ord = ord;
   p->n = n;
   return A3D_OK;
 }
@@ -610,6 +630,11 @@ int a3d_pf_update_dev(a3d_pf* p, const a3d_grid* g, const float* d_x, const floa
   f.roll = (float)roll;  // double -> float parameter of computeCloudWeight
   f.pitch = (float)pitch;
   f.rp0 = (f.roll == 0.0f && f.pitch == 0.0f) ? 1u : 0u;
+  if (g_order == A3D_ORDER_SPATIAL) {
+    const OrderDev o = order_view(p->ord, p->n);
+    launch_order(f, o, p->stream);
+    f.order = o.order;
+  }
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if (!p->capturing) ev = p->timing.take();
   if (ev.first) (void)hipEventRecord(ev.first, p->stream);

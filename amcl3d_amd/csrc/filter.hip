@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kChainThreads = 1024;
 constexpr int kCH = 2048;  // elements per staged block
+constexpr int kRow = kCH + 12;  // LDS row stride: the rows of up to 5 chains start 12 banks apart
 constexpr int kEw = 256;   // elementwise block size
 
 // which chains a chain_kernel launch runs
@@ -39,8 +40,8 @@ __device__ inline float* sums_of(const FilterDev& f) { return f.scratch + sc_sum
 
 template <int K, int PFX>
 __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int which) {
-  __shared__ float4 stage4[2 * K * kCH / 4];
-  __shared__ float4 pstage4[PFX >= 0 ? 2 * kCH / 4 : 1];
+  __shared__ float4 stage4[2 * K * kRow / 4];
+  __shared__ float4 pstage4[PFX >= 0 ? 2 * kCH / 4 + 8 : 1];  // + one junk float4 per chain lane
   const FilterDev& f = fsel(fs, blockIdx.x);
   if (which == kPrefixOnly && f.no_resample) return;
   const uint32_t n = f.n;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
   float acc = 0.f;  // every reference sum starts from 0 (0 + x is x exactly)
   for (int e = tid; e < K * kCH; e += nthr) {
     const int k = e / kCH, j = e % kCH;
-    stage[k * kCH + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
+    stage[k * kRow + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
   }
   __syncthreads();
   for (uint32_t b = 0; b < nblk; ++b) {
@@ -84,14 +85,18 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
 #pragma unroll 4
         for (int e = tid - 64; e < K * kCH; e += nthr - 64) {
           const int k = e / kCH, j = e % kCH;
-          stage[((buf ^ 1) * K + k) * kCH + j] = (nb + j < n) ? src[k][nb + j] : 0.f;
+          stage[((buf ^ 1) * K + k) * kRow + j] = (nb + j < n) ? src[k][nb + j] : 0.f;
         }
       }
       if (PFX >= 0 && b > 0)
         for (int j = tid - 64; j < kCH; j += nthr - 64) prefix[base - kCH + j] = pst[(buf ^ 1) * kCH + j];
     } else if (lane < K) {
-      const float4* row4 = reinterpret_cast<const float4*>(stage + (buf * K + lane) * kCH);
+      const float4* row4 = reinterpret_cast<const float4*>(stage + (buf * K + lane) * kRow);
+      // the prefix lane writes its running sums; the other chain lanes write
+      // to a junk slot of their own, so the store needs no branch
       float4* prow4 = reinterpret_cast<float4*>(pst + buf * kCH);
+      float4* out4 = (PFX >= 0 && lane == PFX) ? prow4 : reinterpret_cast<float4*>(pst + 2 * kCH) + lane;
+      const uint32_t ostep = (PFX >= 0 && lane == PFX) ? 1u : 0u;
       const uint32_t n4 = len >> 2;
       uint32_t q = 0;
       for (; q + 16 <= n4; q += 16) {
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
           o.z = acc;
           acc += v[u].w;
           o.w = acc;
-          if (PFX >= 0 && lane == PFX) prow4[q + u] = o;
+          if (PFX >= 0) out4[(q + u) * ostep] = o;
         }
       }
       const float* row = reinterpret_cast<const float*>(row4);

@@ -17,6 +17,12 @@
 //     smaller.
 //
 // expf is evaluated as (float)exp((double)x), the correctly rounded value.
+//
+// When only prob is built (no dist field), a tile also stops once its radius
+// passes d2_cut, the smallest float d2 whose prob is exactly 0: prob is a
+// non-increasing function of d2, so every cell whose nearest point is not yet
+// found has prob 0 whatever that point is.  prob is unchanged bit for bit;
+// in maps with large empty volumes this bounds the search.
 #include <hipcub/hipcub.hpp>
 
 #include <vector>
@@ -75,6 +81,26 @@ __global__ void bucket_scatter_kernel(const float* xyz, uint32_t n, const uint32
   pz[o] = xyz[3 * (size_t)i + 2];
 }
 
+// prob of a cell whose nearest point is at float squared distance d
+// (PointCloudTools.cpp:137): c1 * expf(-d * d * c2).
+__device__ inline float cell_prob(float d, float c1, float c2) {
+  const float e = -d * d * c2;
+  return c1 * (float)exp((double)e);
+}
+
+// d2_cut: bisection over the bit patterns of non-negative floats (ordered as
+// integers) for the smallest d with cell_prob(d) == 0; +inf gives 0.
+__global__ void prob_cut_kernel(float c1, float c2, float* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t lo = 0u, hi = 0x7f800000u;  // cell_prob(0) = c1 > 0, cell_prob(inf) = 0
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (cell_prob(__uint_as_float(mid), c1, c2) == 0.0f) hi = mid;
+    else lo = mid;
+  }
+  out[0] = __uint_as_float(hi);
+}
+
 __global__ void fill_empty_kernel(float* prob, float* dist, uint64_t cells) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= cells) return;
@@ -103,7 +129,7 @@ __device__ inline double box_gap(const BucketDev& b, const float lo[3], const fl
 
 __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketDev b, float* prob, float* dist,
                                                               float c1, float c2, uint64_t tile0, uint32_t ntx,
-                                                              uint32_t nty, double rmax) {
+                                                              uint32_t nty, double rmax, const float* d2_cut) {
   __shared__ float4 pts[kBatch];
   __shared__ uint32_t blist[kGThreads];
   __shared__ uint32_t bpre[kGThreads + 1];
@@ -145,6 +171,7 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
   // round took by the same test.  Cubes and radii grow, so that union is
   // exactly what every earlier round processed, and every point within r of
   // the tile box lies in a processed bucket.
+  const double cut = d2_cut ? (double)d2_cut[0] : INFINITY;
   double r_prev = -1.0;
   double r = b.h;
   uint32_t plo[3] = {1u, 1u, 1u}, phi[3] = {0u, 0u, 0u};  // previous cube (empty)
@@ -227,9 +254,10 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
     }
     bool ok = true;
     const double lim = r * r * (1.0 - 1e-6);
+    const bool past_cut = lim >= cut;  // unseen points give prob 0, as does best
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      if (cv[c] && !((double)best[c] <= lim)) ok = false;
+      if (cv[c] && !((double)best[c] <= lim) && !past_cut) ok = false;
     // workgroup AND of ok through one LDS word (every thread passes each barrier)
     if (tid == 0) notdone = 0u;
     __syncthreads();
@@ -244,15 +272,16 @@ __global__ __launch_bounds__(kGThreads) void grid_tile_kernel(GridDev g, BucketD
       phi[a] = bhi[a];
     }
     r_prev = r;
-    r = r * 2.0;
+    // the radius at which the prob cut applies, if that comes before 2r
+    const double rc = sqrt(cut / (1.0 - 1e-6)) * (1.0 + 1e-9);
+    r = (rc > r && rc < 2.0 * r) ? rc : 2.0 * r;
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if (!cv[c]) continue;
     const uint64_t idx = (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)(iz0 + 2 * c) * g.step_z;
     const float d = best[c];
-    const float e = -d * d * c2;
-    prob[idx] = c1 * (float)exp((double)e);
+    prob[idx] = cell_prob(d, c1, c2);
     if (dist) dist[idx] = d;
   }
 }
@@ -280,7 +309,7 @@ int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_p
   if (nbk >= 0xfffffff0ull || n >= 0xfffffff0ull)
     return fail(A3D_ERR_ARG, "computeGrid: too many buckets or points for one build");
   uint32_t *cnt = nullptr, *start = nullptr, *key = nullptr;
-  float *px = nullptr, *py = nullptr, *pz = nullptr;
+  float *px = nullptr, *py = nullptr, *pz = nullptr, *cut = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   int rc = A3D_OK;
@@ -291,6 +320,7 @@ int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_p
     (void)hipFree(px);
     (void)hipFree(py);
     (void)hipFree(pz);
+    (void)hipFree(cut);
     (void)hipFree(tmp);
   };
 #define A3D_TRY(call)              \
@@ -323,10 +353,15 @@ int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_p
   double rmax = 0.0;
   for (int a = 0; a < 3; ++a) rmax += (b.nb[a] * b.h + 1.0) * (b.nb[a] * b.h + 1.0);
   rmax = std::sqrt(rmax) * 2.0;
+  if (!d_dist) {  // prob only: the search may stop where prob reaches 0
+    A3D_TRY(hipMalloc(&cut, sizeof(float)));
+    prob_cut_kernel<<<1, 64, 0, s>>>(c1, c2, cut);
+    A3D_TRY(hipGetLastError());
+  }
   const uint64_t chunk = 1ull << 22;
   for (uint64_t t0 = 0; t0 < ntiles; t0 += chunk) {
     const uint32_t nblk = (uint32_t)std::min<uint64_t>(chunk, ntiles - t0);
-    grid_tile_kernel<<<nblk, kGThreads, 0, s>>>(g, b, d_prob, d_dist, c1, c2, t0, ntx, nty, rmax);
+    grid_tile_kernel<<<nblk, kGThreads, 0, s>>>(g, b, d_prob, d_dist, c1, c2, t0, ntx, nty, rmax, cut);
     A3D_TRY(hipGetLastError());
   }
   A3D_TRY(hipStreamSynchronize(s));

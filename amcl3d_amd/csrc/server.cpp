@@ -6,6 +6,9 @@
 // Philox keyed by the session seed and indexed by the session's step, so a
 // session's results are the same whichever requests share its batches.
 // Sessions are admitted while their device bytes fit in the HBM budget.
+// With latest_only, a session keeps at most one queued request: a newer scan
+// supersedes the queued one (as the reference's queue-size-1 subscriber
+// drops stale scans, Node.cpp:60), and the two odometry increments compose.
 //
 // A worker thread forms batches from the queue (arrival order, at most one
 // request per session in flight, bounded by sessions and evaluations per
@@ -18,6 +21,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -79,6 +83,16 @@ struct Request {
   int resample = 0;
   double submit_ms = 0;
 };
+
+// Odometry increment a followed by b (both relative to the pose before them),
+// as Node.cpp:147-151 would read it from lastupdatebase^-1 * base.
+void compose_increment(const double a[4], const double b[4], double out[4]) {
+  const double c = std::cos(a[3]), s = std::sin(a[3]);
+  out[0] = a[0] + (c * b[0] - s * b[1]);
+  out[1] = a[1] + (s * b[0] + c * b[1]);
+  out[2] = a[2] + b[2];
+  out[3] = a[3] + b[3];
+}
 
 // Device and pinned buffers of one in-flight batch.
 struct Slot {
@@ -311,6 +325,8 @@ void a3d_server::retire(Slot& s) {
     res.latency_ms = t - r.submit_ms;
     res.submit_ms = r.submit_ms;
     res.evals = (uint64_t)sessions[r.session].n * r.m;
+    std::memcpy(res.deltas, r.deltas, 32);
+    res.superseded = 0;
     results.push_back(res);
     sessions[r.session].busy = false;
     st_evals += res.evals;
@@ -504,6 +520,38 @@ int a3d_server_open_session(a3d_server* S, int map_id, int num_particles, const 
   return A3D_OK;
 }
 
+int a3d_server_open_session_particles(a3d_server* S, int map_id, const float* p7, size_t num_particles,
+                                      uint64_t seed, int* session) {
+  if (!S || !p7 || !session || num_particles == 0 || num_particles > 0x7fffffffull)
+    return fail(A3D_ERR_ARG, "a3d_server_open_session_particles: bad argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (map_id < 0 || map_id >= (int)S->maps.size())
+    return fail(A3D_ERR_ARG, "a3d_server_open_session_particles: bad map");
+  Session se;
+  se.map = map_id;
+  se.n = (uint32_t)num_particles;
+  se.seed = seed;
+  int rc = alloc_session(S, se);
+  if (rc) return rc;
+  const size_t n = se.n;
+  std::vector<float> soa(7 * n);
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 7; ++k) soa[k * n + i] = p7[7 * i + k];
+  const float zero[4] = {0.f, 0.f, 0.f, 0.f};
+  const bool ok = hipMemcpyAsync(se.state, soa.data(), 7 * n * 4, hipMemcpyHostToDevice, S->admin) == hipSuccess &&
+                  hipMemcpyAsync(se.mean, zero, 16, hipMemcpyHostToDevice, S->admin) == hipSuccess &&
+                  hipStreamSynchronize(S->admin) == hipSuccess;
+  if (!ok) {
+    free_session(S, se);
+    return fail(A3D_ERR_HIP, "a3d_server_open_session_particles: upload failed");
+  }
+  se.open = true;
+  se.step = 1;  // step 0's draws belong to init, which the caller's particles replace
+  S->sessions.push_back(se);
+  *session = (int)S->sessions.size() - 1;
+  return A3D_OK;
+}
+
 int a3d_server_close_session(a3d_server* S, int session) {
   if (!S) return fail(A3D_ERR_ARG, "a3d_server_close_session: null server");
   std::unique_lock<std::mutex> lk(S->mu);
@@ -547,6 +595,26 @@ int a3d_server_submit(a3d_server* S, int session, const double mods[4], const do
     r.ticket = S->next_ticket++;
     r.submit_ms = now_ms();
     if (ticket) *ticket = r.ticket;
+    if (S->cfg.latest_only) {
+      for (auto it = S->pending.begin(); it != S->pending.end(); ++it) {
+        if (it->session != session) continue;
+        // the queued request never runs: report it and fold its motion in
+        a3d_result old{};
+        old.ticket = it->ticket;
+        old.session = session;
+        old.latency_ms = r.submit_ms - it->submit_ms;
+        old.submit_ms = it->submit_ms;
+        std::memcpy(old.deltas, it->deltas, 32);
+        old.superseded = 1;
+        S->results.push_back(old);
+        double d[4];
+        compose_increment(it->deltas, r.deltas, d);
+        std::memcpy(r.deltas, d, 32);
+        S->pending.erase(it);
+        S->inflight_requests--;
+        break;  // at most one queued request per session
+      }
+    }
     S->inflight_requests++;
     S->pending.push_back(std::move(r));
   }

@@ -3,9 +3,19 @@
 // (ParticleFilter.cpp:224-244) of ParticleFilter::update's first loop
 // (ParticleFilter.cpp:129-153).
 //
-// Mapping.  One wavefront owns a block of up to PB particles of one filter
-// and walks the whole cloud in chunks of 64 points, lane j holding point
-// c0+j.  For each particle i of the block the 64 lanes transform their points
+// Two mappings of (particle, point) pairs to lanes give identical bits; the
+// caller picks one with a3d_set_mapping (an autotuner times both).
+//
+// Mapping 0, particle lanes (lanes_*_kernel).  A wavefront carries P = 64/Q
+// particles; lane l holds particle l mod P and, per step, point j0 + l div P,
+// so the wavefront advances Q consecutive points of the cloud per step and
+// the P lanes of one point gather for P nearby poses.  The Q values of one
+// particle reach its lane by shuffles and are added in point order, so the
+// sum is the reference's sequential float sum.
+//
+// Mapping 1, point lanes (weight_*_kernel).  One wavefront owns a block of up
+// to PB particles of one filter and walks the whole cloud in chunks of 64
+// points, lane j holding point c0+j.  For each particle i of the block the 64 lanes transform their points
 // with particle i's pose (uniform values taken from lane i with readlane) and
 // gather prob from the grid: 64 points of one particle per gather, so the
 // addresses of one instruction are as close in the map as the points are in
@@ -18,6 +28,15 @@
 
 namespace a3d {
 namespace {
+
+// Workgroups are dealt to the 8 XCDs round-robin by index.  Single-filter
+// launches renumber them so that each XCD takes one contiguous run of the
+// (spatially ordered) particle blocks, and its L2 holds the cells of one
+// region of pose space.  A bijection of [0, nb).
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / 8, r = nb % 8, x = b % 8, s = b / 8;
+  return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
 
 constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16-byte rows)
 constexpr int kWaves = 4;      // wavefronts per workgroup
@@ -105,7 +124,8 @@ __device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, 
                              float* out) {
   const int lane = lane_id();
   const bool has = (uint32_t)lane < count;
-  const uint32_t pi = p0 + (has ? (uint32_t)lane : 0u);
+  const uint32_t pk = p0 + (has ? (uint32_t)lane : 0u);
+  const uint32_t pi = f.order ? f.order[pk] : pk;
   float tx = 0.f, ty = 0.f, tz = 0.f, ta = 0.f;
   if (has) {
     tx = f.x[pi];
@@ -263,7 +283,8 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   const int lane = lane_id();
   const int ps = lane % P, po = lane / P;
   const bool has = (uint32_t)ps < count;
-  const uint32_t pi = p0 + (has ? (uint32_t)ps : 0u);
+  const uint32_t pk = p0 + (has ? (uint32_t)ps : 0u);
+  const uint32_t pi = f.order ? f.order[pk] : pk;
   float tx = 0.f, ty = 0.f, tz = 0.f, ta = 0.f;
   if (has) {
     tx = f.x[pi];
@@ -300,7 +321,7 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
 template <int Q, bool IDX64>
 __global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
   constexpr uint32_t P = 64 / Q;
-  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   if (p0 >= f.n) return;
   weight_lanes<Q, IDX64, true>(g, f, p0, min(P, f.n - p0), nullptr);
@@ -352,7 +373,7 @@ template <int PB, bool IDX64>
 __global__ __launch_bounds__(kThreads) void weight_single_kernel(GridDev g, FilterDev f) {
   __shared__ float4 tiles4[kWaves * PB * kTS / 4];
   float* tile = reinterpret_cast<float*>(tiles4) + (threadIdx.x >> 6) * PB * kTS;
-  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * PB;
   if (p0 >= f.n) return;
   const uint32_t count = min((uint32_t)PB, f.n - p0);

@@ -5,7 +5,7 @@ Under ``rocprofv3 --pmc ...`` the copy kernel's FETCH_SIZE / WRITE_SIZE
 (it reads and writes ``--copy-mib`` MiB) calibrate the counters that the
 cloud-weight dispatches report.
 
-    python -m amcl3d_amd.profile_run --updates 3 --copy-mib 1024 --shape 0,16
+    python -m amcl3d_amd.profile_run --updates 3 --copy-mib 1024 --shape 0,16,1
 """
 import argparse
 import sys
@@ -21,7 +21,7 @@ def main(argv=None):
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--points", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--shape", default="", help="cloud-weight kernel shape 'MAPPING,PPW' (default: the library's)")
+    ap.add_argument("--shape", default="", help="cloud-weight kernel shape 'MAPPING,PPW,ORDER' (default: the library's)")
     args = ap.parse_args(argv)
 
     import amcl3d_amd
@@ -29,9 +29,9 @@ def main(argv=None):
 
     amcl3d_amd.load()
     if args.shape:
-        mapping, ppw = (int(v) for v in args.shape.split(","))
-        amcl3d_amd.set_mapping(mapping)
-        amcl3d_amd.set_tile(ppw)
+        from amcl3d_amd import autotune
+
+        autotune.set_shape(tuple(int(v) for v in args.shape.split(",")))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     wl = benchlib.large_filter_workload(amcl3d_amd, W, args.updates, args.seed, args.particles, args.points, 8,
                                         log=log)

@@ -26,9 +26,9 @@ def session_draws(seed, step, n):
 
 class Server:
     def __init__(self, hbm_budget_bytes=64 << 30, max_batch_evals=8_000_000_000, max_batch_sessions=512,
-                 max_inflight=3):
+                 max_inflight=3, latest_only=False):
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_batch_evals), int(max_batch_sessions),
-                                int(max_inflight))
+                                int(max_inflight), 1 if latest_only else 0)
         h = ctypes.c_void_p()
         _lib.call("a3d_server_create", ctypes.byref(cfg), ctypes.byref(h))
         self._h = h.value
@@ -58,6 +58,15 @@ class Server:
         _lib.call("a3d_server_open_session", self._h, int(map_id), int(num_particles),
                   _lib.f32(pose), _lib.f32(devs), int(seed), ctypes.byref(sid))
         self._n[sid.value] = int(num_particles)
+        return sid.value
+
+    def open_session_particles(self, map_id, particles, seed):
+        """A session from given (n, 7) particles (x, y, z, a, w, wp, wr)."""
+        p = _lib.f32(particles).reshape(-1, 7)
+        sid = ctypes.c_int()
+        _lib.call("a3d_server_open_session_particles", self._h, int(map_id), p, p.shape[0], int(seed),
+                  ctypes.byref(sid))
+        self._n[sid.value] = int(p.shape[0])
         return sid.value
 
     def close_session(self, sid):

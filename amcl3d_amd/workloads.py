@@ -131,6 +131,20 @@ def building_map(seed, extent=(100.0, 100.0, 30.0), res=0.05, floor_height=3.0, 
     return BoxMap(np.zeros(3), ex, res, boxes)
 
 
+def city_map(seed, extent=(200.0, 200.0, 40.0), res=0.05, n_boxes=85, footprint=(4.0, 16.0), height=(3.0, 35.0)):
+    """A city block: a ground slab and n_boxes buildings (boxes) with footprint
+    sides U(footprint) m and heights U(height) m, standing on the ground."""
+    rng = np.random.default_rng(seed)
+    ex = np.asarray(extent, np.float64)
+    boxes = [[0, 0, 0, ex[0], ex[1], res]]
+    for _ in range(n_boxes):
+        w = rng.uniform(*footprint, 2)
+        h = min(rng.uniform(*height), ex[2] - 2 * res)
+        c = rng.uniform(w / 2, ex[:2] - w / 2)
+        boxes.append([c[0] - w[0] / 2, c[1] - w[1] / 2, res, c[0] + w[0] / 2, c[1] + w[1] / 2, res + h])
+    return BoxMap(np.zeros(3), ex, res, boxes)
+
+
 def storey_view(bmap, z, floor_height):
     """The boxes a sensor at height z sees: its storey's floor, ceiling, walls, obstacles."""
     k = np.floor(z / floor_height)

@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--shape", default="auto",
-                    help="cloud-weight kernel shape 'MAPPING,PPW' (e.g. 0,32) or 'auto' to time every shape")
+                    help="cloud-weight kernel shape 'MAPPING,PPW,ORDER' (e.g. 0,16,1) or 'auto' to time every shape")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the filter path has no multi-GPU form "
@@ -166,6 +166,7 @@ def main():
             "bound": "hbm",
             "kernel": "cloud weight (%s)" % ("lanes_single_kernel" if amcl3d_amd.get_mapping() == 0
                                              else "weight_single_kernel"),
+            "kernel_note": "the particle visiting order (4 small kernels, outside the HIP events) is in ms_per_step",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -187,7 +188,7 @@ def main():
             from amcl3d_amd import benchlib
 
             out["extras"] = benchlib.extras(amcl3d_amd, W, plain_value=value,
-                                            building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5))
+                                            building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5), log=log)
         except Exception as e:  # extras never hide the main line
             out["extras"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

@@ -167,6 +167,14 @@ int a3d_set_tile(int particles_per_wave);
 int a3d_get_tile(int* particles_per_wave);
 int a3d_set_mapping(int mapping);
 int a3d_get_mapping(int* mapping);
+/* Order in which the cloud-weight kernel visits a filter's particles (the
+ * particles themselves stay in index order; results are identical):
+ * A3D_ORDER_INDEX, or A3D_ORDER_SPATIAL, a counting sort by a Morton code of
+ * (x, y, z, yaw) over the particles' bounding box, computed on the device
+ * before each single-filter update. */
+enum { A3D_ORDER_INDEX = 0, A3D_ORDER_SPATIAL = 1 };
+int a3d_set_order(int order);
+int a3d_get_order(int* order);
 
 /* -------------------------------------------------------------------- Fleet */
 /* Many independent filters on one grid, updated by one persistent launch.
@@ -208,6 +216,11 @@ typedef struct {
   uint64_t max_batch_evals;   /* particle.point evaluations per launch */
   int max_batch_sessions;     /* sessions per launch */
   int max_inflight;           /* launches in flight (streams) */
+  int latest_only;            /* != 0: a session keeps at most one queued request; a
+                                 newer one supersedes it (the reference's scan
+                                 subscriber has queue size 1, Node.cpp:60), and
+                                 the superseded odometry increment is composed
+                                 into the newer one */
 } a3d_server_config;
 
 typedef struct {
@@ -216,7 +229,11 @@ typedef struct {
   float mean[4];
   double latency_ms;    /* submit -> result on the host */
   double submit_ms;     /* on the server clock */
-  uint64_t evals;
+  uint64_t evals;       /* 0 for a superseded request */
+  double deltas[4];     /* odometry increment the update applied (its own, composed
+                           after those of the requests it superseded) */
+  int superseded;       /* 1: replaced by a newer request of the session before
+                           it ran; mean is not set */
 } a3d_result;
 
 int a3d_server_create(const a3d_server_config* cfg, a3d_server** out);
@@ -227,9 +244,19 @@ int a3d_server_add_map(a3d_server* s, const a3d_grid* g, int* map_id);
  * HBM budget is exhausted. */
 int a3d_server_open_session(a3d_server* s, int map_id, int num_particles, const float pose[4], const float devs[4],
                             uint64_t seed, int* session);
+/* Opens a session from given particles (struct Particle[n]), e.g. a global
+ * localisation set drawn uniformly over the map's free space; the session's
+ * draws then start at step 1, as after a3d_server_open_session. */
+int a3d_server_open_session_particles(a3d_server* s, int map_id, const float* p7, size_t num_particles,
+                                      uint64_t seed, int* session);
 int a3d_server_close_session(a3d_server* s, int session);
 /* Queues predict + update (+ resample if resample != 0) for a session.  The
- * cloud and ranges are copied at submit. */
+ * cloud and ranges are copied at submit.  deltas is the odometry increment
+ * since the session's previous request: (x, y, z) in the frame of the
+ * previous pose and the yaw change.  When a request supersedes a queued one
+ * (latest_only) the increments compose as poses do: x = a.x + cos(a.yaw)*b.x
+ * - sin(a.yaw)*b.y, y = a.y + sin(a.yaw)*b.x + cos(a.yaw)*b.y, z = a.z + b.z,
+ * yaw = a.yaw + b.yaw. */
 int a3d_server_submit(a3d_server* s, int session, const double mods[4], const double deltas[4], const float* x,
                       const float* y, const float* z, size_t m, const float* ranges, size_t nb, double alpha,
                       double sigma, double roll, double pitch, int resample, uint64_t* ticket);

@@ -94,6 +94,26 @@ def test_grid_empty_and_tiny(gpu):
     assert np.array_equal(d, od.reshape(-1)) and np.array_equal(p, op.reshape(-1))
 
 
+def test_grid_prob_only_sparse_map(gpu):
+    """Without the dist field the search stops where prob reaches 0; on a
+    sparse map (most cells farther than that from any point) prob is still
+    bit-exact, and the same as a build that keeps dist."""
+    bm = W.BoxMap(np.zeros(3), np.array([10.0, 8.0, 4.0]), 0.05,
+                  [[1.0, 1.0, 0.0, 1.6, 1.4, 0.9], [7.2, 5.5, 1.0, 7.5, 6.9, 3.1], [4.0, 0.3, 3.7, 4.2, 0.6, 3.9]])
+    pts = bm.points()
+    _, oprob = oracle.compute_grid(pts, bm.min, bm.max, bm.res, 0.05, exp_mode=1)
+    g = gpu.Grid3d()
+    _say("sparse map prob-only build", bm.size, pts.shape)
+    g.open(pts, bm.min, bm.max, bm.res, 0.05)
+    p = g.read_cells()
+    assert np.array_equal(p.view(np.int32), oprob.reshape(-1).view(np.int32))
+    assert (p == 0).mean() > 0.5  # most of the map is beyond the cut
+    g2 = gpu.Grid3d()
+    g2.open(pts, bm.min, bm.max, bm.res, 0.05, keep_dist=True)
+    p2, _ = g2.read_cells(dist=True)
+    assert np.array_equal(p.view(np.int32), p2.view(np.int32))
+
+
 # --------------------------------------------------------- computeCloudWeight
 @pytest.fixture(params=[(0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)],
                 ids=lambda mt: "map%d-ppw%d" % mt)
@@ -238,6 +258,30 @@ def test_full_step_bit_exact(gpu, room, seed):
     ref_idx = oracle.resample_indices(op[:, 4], u01, 0)
     f64_idx = oracle.resample_indices(op[:, 4], u01, 1)
     _say("seed", seed, "resample indices differing between float and FP64 prefix:", int((ref_idx != f64_idx).sum()))
+
+
+@pytest.mark.parametrize("mapping,ppw", [(0, 64), (0, 16), (1, 32)])
+def test_update_visiting_order_invariance(gpu, room, mapping, ppw):
+    """The particle visiting order (index or spatial) and the kernel shape
+    change nothing: the update stays bit-exact."""
+    pose, cloud, ranges, parts, z4, u01 = _full_step_case(gpu, room, 41, n=3001, out_of_map=5)
+    op, om = oracle.pf_update(parts, room["ogrid"], cloud, ranges, 0.8, 0.1)
+    saved = gpu.get_mapping(), gpu.get_tile(), gpu.get_order()
+    try:
+        gpu.set_mapping(mapping)
+        gpu.set_tile(ppw)
+        for order in (gpu.ORDER_INDEX, gpu.ORDER_SPATIAL):
+            _say("update with mapping", mapping, "ppw", ppw, "order", order)
+            gpu.set_order(order)
+            pf = gpu.ParticleFilter()
+            pf.set_particles(parts)
+            pf.update(room["grid"], cloud, ranges, 0.8, 0.1)
+            assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32)), order
+            assert np.array_equal(pf.getMean().view(np.int32), om.view(np.int32)), order
+    finally:
+        gpu.set_mapping(saved[0])
+        gpu.set_tile(saved[1])
+        gpu.set_order(saved[2])
 
 
 def test_update_roll_pitch_and_no_map(gpu, room):

@@ -126,6 +126,87 @@ def test_mixed_maps_in_one_batch(gpu, room_grid):
     assert np.array_equal(pf.particles().view(np.int32), mixed[1].view(np.int32))
 
 
+def test_latest_only_supersedes_and_composes(gpu, room_grid):
+    """latest_only: a queued request is replaced by the session's newer one
+    (the reference's scan subscriber keeps one message, Node.cpp:60) and the
+    two odometry increments compose; the session equals the plain filter
+    driven with the increments that ran."""
+    from amcl3d_amd.serving import Server, session_draws
+
+    bm, grid = room_grid
+    sessions = _sessions(bm, 2, 31)
+    # a large first session keeps the GPU busy so the small one's requests queue up
+    n0, pose0, seed0, steps0 = sessions[0]
+    n1, pose1, seed1, steps1 = sessions[1]
+    srv = Server(max_inflight=1, max_batch_sessions=1, latest_only=True)
+    mid = srv.add_map(grid)
+    s0 = srv.open_session(mid, 60000, pose0, [0.3, 0.3, 0.1, 0.2], seed0)
+    s1 = srv.open_session(mid, n1, pose1, [0.3, 0.3, 0.1, 0.2], seed1)
+    big = W.sensor_cloud(bm, pose0, 20000, 0.05, np.random.default_rng(3))
+    bigxyz = tuple(np.ascontiguousarray(big[:, j]) for j in range(3))
+    mods = [0.1, 0.1, 0.1, 0.3]
+    rng = np.random.default_rng(8)
+    sent = {}
+    for k in range(9):
+        srv.submit(s0, mods, [0.01, 0.0, 0.0, 0.0], bigxyz, None, 0.8, 0.1)
+        xyz, rg, _, _ = steps1[k % 3]
+        d = rng.uniform(-0.1, 0.1, 4)
+        rs = bool(k % 2)
+        t = srv.submit(s1, mods, d, xyz, rg, 0.8, 0.1, resample=rs)
+        sent[t] = (xyz, rg, rs)
+    srv.drain()
+    res = sorted((r for r in srv.poll(1000) if r.session == s1), key=lambda r: r.ticket)
+    got = srv.particles(s1)
+    srv.close()
+    assert len(res) == 9
+    ran = [r for r in res if not r.superseded]
+    print("latest_only: %d of 9 requests superseded" % (9 - len(ran)), flush=True)
+    assert ran and ran[-1].ticket == max(sent)  # the newest request always runs
+    pf = gpu.ParticleFilter()
+    z0, _ = session_draws(seed1, 0, n1)
+    pf.init(n1, *pose1, 0.3, 0.3, 0.1, 0.2, z0)
+    for k, r in enumerate(ran):
+        xyz, rg, rs = sent[r.ticket]
+        z4, u01 = session_draws(seed1, k + 1, n1)
+        pf.predict(*mods, *list(r.deltas), z4)
+        pf.update(grid, np.stack(xyz, 1), rg, 0.8, 0.1)
+        if rs:
+            pf.resample(u01)
+    assert np.array_equal(pf.particles().view(np.int32), got.view(np.int32))
+
+
+def test_session_from_particles(gpu, room_grid):
+    """A session opened from given particles (global localisation) equals the
+    plain filter started from the same particles."""
+    from amcl3d_amd.serving import Server, session_draws
+
+    bm, grid = room_grid
+    rng = np.random.default_rng(12)
+    n = 4000
+    parts = np.zeros((n, 7), F32)
+    parts[:, :3] = rng.uniform(bm.min + 0.5, bm.max - 0.5, (n, 3))
+    parts[:, 3] = rng.uniform(-np.pi, np.pi, n)
+    parts[:, 4] = 1.0 / n
+    (_, pose, _, steps), = _sessions(bm, 1, 13)
+    srv = Server(max_inflight=2)
+    mid = srv.add_map(grid)
+    sid = srv.open_session_particles(mid, parts, 77)
+    for xyz, rg, d, rs in steps:
+        srv.submit(sid, [0.1, 0.1, 0.1, 0.3], d, xyz, rg, 0.8, 0.1, resample=rs)
+    srv.drain()
+    got = srv.particles(sid)
+    srv.close()
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    for k, (xyz, rg, d, rs) in enumerate(steps):
+        z4, u01 = session_draws(77, k + 1, n)
+        pf.predict(0.1, 0.1, 0.1, 0.3, *d, z4)
+        pf.update(grid, np.stack(xyz, 1), rg, 0.8, 0.1)
+        if rs:
+            pf.resample(u01)
+    assert np.array_equal(pf.particles().view(np.int32), got.view(np.int32))
+
+
 def test_admission_by_hbm_budget(gpu, room_grid):
     from amcl3d_amd.serving import Server
 
