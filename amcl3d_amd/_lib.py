@@ -77,6 +77,8 @@ SIGNATURES = {
     "a3d_grid_read_cells": [_vp, _u64, _u64, _vp, _vp],
     "a3d_grid_is_into_map": [_vp, _f, _f, _f, _p(_i)],
     "a3d_grid_cloud_weight": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _sz, _f, _f, _vp],
+    "a3d_voxel_grid": [_vp, _sz, _sz, _f, _vp, _vp, _vp, _p(_sz)],
+    "a3d_voxel_grid_dev": [_vp, _sz, _sz, _f, _vp, _vp, _vp, _p(_sz)],
     "a3d_pf_create": [_p(_vp)],
     "a3d_pf_destroy": [_vp],
     "a3d_pf_init": [_vp, _i, _f, _f, _f, _f, _f, _f, _f, _f, _vp],
@@ -279,3 +281,15 @@ class DeviceBuffer:
                 _LIB.a3d_buffer_free(self.ptr)
         except Exception:
             pass
+
+
+def voxel_grid(points, leaf):
+    """pcl::VoxelGrid<PointXYZ> on the device (Node.cpp:130-141): (k, 3) float32
+    centroids in voxel order.  points: (n, 3) or (n, 4) float32 (PointXYZ)."""
+    p = f32(points)
+    p = p.reshape(p.shape[0], -1) if p.size else p.reshape(0, 3)
+    n = p.shape[0]
+    out = np.empty((3, max(n, 1)), np.float32)
+    k = ctypes.c_size_t()
+    call("a3d_voxel_grid", p, n, p.shape[1] * 4, float(leaf), out[0], out[1], out[2], ctypes.byref(k))
+    return np.ascontiguousarray(out[:, :k.value].T)

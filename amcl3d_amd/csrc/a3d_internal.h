@@ -161,6 +161,10 @@ void launch_order(const FilterDev& f, const OrderDev& o, hipStream_t s);
 int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_prob, float* d_dist,
                       double sensor_dev, hipStream_t s);
 
+// pcl::VoxelGrid as Node.cpp:130-141 applies it; synchronous on stream s.
+int voxel_grid_device(const void* d_pts, size_t n, size_t stride, float leaf, float* d_x, float* d_y, float* d_z,
+                      size_t* out_n, hipStream_t s);
+
 // Counter-based draws for served sessions (Philox4x32-10 + Box-Muller).
 // Writes 4 normals per particle to each filter's z4 and one uniform per
 // filter to d_u01 (may be null).

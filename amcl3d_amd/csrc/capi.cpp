@@ -177,6 +177,50 @@ int a3d_buffer_copy(void* dst, const void* src, size_t bytes) {
   return A3D_OK;
 }
 
+static int voxel_args(size_t n, size_t stride, float leaf, const void* pts, const float* x, const float* y,
+                      const float* z, size_t* out_n) {
+  if (!out_n || (n && (!pts || !x || !y || !z))) return fail(A3D_ERR_ARG, "a3d_voxel_grid: null argument");
+  if (stride < 12 || stride % 4 != 0) return fail(A3D_ERR_ARG, "a3d_voxel_grid: stride must be >= 12, a multiple of 4");
+  if (!(leaf > 0.0f) || !std::isfinite(leaf)) return fail(A3D_ERR_ARG, "a3d_voxel_grid: leaf must be positive");
+  if (n > 0x7fffffffull) return fail(A3D_ERR_ARG, "a3d_voxel_grid: too many points");
+  return A3D_OK;
+}
+
+int a3d_voxel_grid_dev(const void* d_points, size_t n, size_t stride, float leaf, float* d_x, float* d_y, float* d_z,
+                       size_t* out_n) {
+  int rc = voxel_args(n, stride, leaf, d_points, d_x, d_y, d_z, out_n);
+  if (rc) return rc;
+  return voxel_grid_device(d_points, n, stride, leaf, d_x, d_y, d_z, out_n, nullptr);
+}
+
+int a3d_voxel_grid(const void* points, size_t n, size_t stride, float leaf, float* out_x, float* out_y, float* out_z,
+                   size_t* out_n) {
+  int rc = voxel_args(n, stride, leaf, points, out_x, out_y, out_z, out_n);
+  if (rc) return rc;
+  *out_n = 0;
+  if (n == 0) return A3D_OK;
+  void* d_in = nullptr;
+  float* d_out = nullptr;
+  if (hipMalloc(&d_in, n * stride) != hipSuccess || hipMalloc(&d_out, 3 * n * 4) != hipSuccess) {
+    (void)hipFree(d_in);
+    return fail(A3D_ERR_NOMEM, "a3d_voxel_grid: device allocation failed");
+  }
+  size_t k = 0;
+  rc = hip_ok(hipMemcpy(d_in, points, n * stride, hipMemcpyHostToDevice), "a3d_voxel_grid: upload") ? A3D_OK
+                                                                                                    : A3D_ERR_HIP;
+  if (!rc) rc = voxel_grid_device(d_in, n, stride, leaf, d_out, d_out + n, d_out + 2 * n, &k, nullptr);
+  if (!rc && k) {
+    const bool ok = hipMemcpy(out_x, d_out, k * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+                    hipMemcpy(out_y, d_out + n, k * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+                    hipMemcpy(out_z, d_out + 2 * n, k * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) rc = fail(A3D_ERR_HIP, "a3d_voxel_grid: download failed");
+  }
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  if (!rc) *out_n = k;
+  return rc;
+}
+
 int a3d_set_tile(int particles_per_wave) {
   if (particles_per_wave != 16 && particles_per_wave != 32 && particles_per_wave != 64)
     return fail(A3D_ERR_ARG, "a3d_set_tile: particles per wavefront must be 16, 32 or 64");

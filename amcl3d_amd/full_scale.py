@@ -1,13 +1,12 @@
 """Full-scale serving with a device-side map build (one MI355X).
 
-    python -m amcl3d_amd.full_scale [--duration 5] [--no-check]
+    python -m amcl3d_amd.full_scale [--duration 5]
 
 Map: a seeded city block of 200 x 200 x 40 m at 0.05 m (4000 x 4000 x 800 =
 1.28e10 cells, 51.2 GB of FP32 prob, 64-bit cell indices) whose ~49M
 occupied points (ground slab and 85 buildings) go through computeGrid on the
-device (PointCloudTools.cpp:84-149).  The build is checked on sampled cells
-against the oracle's exact float nearest distance (bit-exact prob expected)
-and against an FP64 nearest distance (scipy cKDTree; error stated).
+device (PointCloudTools.cpp:84-149).  tests/test_gpu_full_scale.py checks the
+same build on sampled cells.
 
 Sessions: 64 global-localisation sessions of 1,000,000 particles drawn
 uniformly over the free space and 2,048 tracking sessions of U(5,000, 20,000)
@@ -49,59 +48,17 @@ def free_particles(bm, n, rng):
     return parts
 
 
-def check_grid(grid, pts, bm, sensor_dev, rng, nsample):
-    """prob of sampled cells against the oracle (float nearest distance, as
-    PCL's KdTreeFLANN computes it) and an FP64 nearest distance."""
-    sys.path.insert(0, ROOT)
-    import oracle
-    from scipy.spatial import cKDTree
-
-    sz = np.asarray(grid.size, np.int64)
-    # half random cells, half cells next to occupied points (where prob > 0)
-    lin_r = rng.integers(0, int(np.prod(sz)), nsample // 2)
-    near = pts[rng.integers(0, pts.shape[0], nsample - nsample // 2)].astype(np.float64)
-    near += rng.uniform(-0.6, 0.6, near.shape)
-    ijk = np.clip(np.floor((near - bm.min) / bm.res).astype(np.int64), 0, sz - 1)
-    lin_n = ijk[:, 0] + sz[0] * (ijk[:, 1] + sz[1] * ijk[:, 2])
-    lin = np.concatenate([lin_r, lin_n])
-    ix, iy, iz = lin % sz[0], (lin // sz[0]) % sz[1], lin // (sz[0] * sz[1])
-    q = np.stack([(bm.min[0] + ix * bm.res), (bm.min[1] + iy * bm.res), (bm.min[2] + iz * bm.res)], -1)
-    q32 = q.astype(np.float32)
-    got = np.empty(lin.size, np.float32)
-    order = np.argsort(lin)
-    for k in order:  # cells one by one from the device (sampled, not the whole 51 GB)
-        got[k] = grid.read_cells(int(lin[k]), 1)[0]
-    t0 = time.time()
-    d2 = oracle.nearest_d2(pts, bm.min, bm.max, bm.res, q32)
-    log("oracle nearest distances for %d cells (%.1fs)" % (lin.size, time.time() - t0))
-    c1 = np.float32(1.0 / (sensor_dev * np.sqrt(2 * np.pi)))
-    c2 = np.float32(1.0 / (2.0 * sensor_dev * sensor_dev))
-    e = (-d2 * d2 * c2).astype(np.float32)
-    ref = (c1 * np.exp(e.astype(np.float64)).astype(np.float32)).astype(np.float32)
-    ulps = np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32))
-    t0 = time.time()
-    d64, _ = cKDTree(pts.astype(np.float64)).query(q32.astype(np.float64))
-    log("FP64 nearest distances (%.1fs)" % (time.time() - t0))
-    p64 = c1.astype(np.float64) * np.exp(-(d64 ** 2) ** 2 * float(c2))
-    err = np.abs(got.astype(np.float64) - p64)
-    return {"cells_checked": int(lin.size), "prob_exact_vs_oracle": float((ulps == 0).mean()),
-            "prob_max_ulp_vs_oracle": int(ulps.max()), "prob_max_abs_err_vs_fp64": float(err.max()),
-            "prob_peak": float(c1), "nonzero_fraction_of_checked": float((got > 0).mean())}
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--duration", type=float, default=5.0)
     ap.add_argument("--rate", type=float, default=10.0)
     ap.add_argument("--global-sessions", type=int, default=64)
     ap.add_argument("--tracking-sessions", type=int, default=2048)
-    ap.add_argument("--check-cells", type=int, default=4000)
-    ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
     sys.path.insert(0, ROOT)
     import amcl3d_amd
-    from amcl3d_amd import benchlib, workloads as W
+    from amcl3d_amd import workloads as W
 
     amcl3d_amd.load()
     rng = np.random.default_rng(args.seed)
@@ -121,9 +78,6 @@ def main(argv=None):
     out["map"] = {"cells": int(grid.cells), "points": int(pts.shape[0]), "prob_bytes": int(grid.cells) * 4,
                   "build_s": build_s, "cells_per_s": grid.cells / build_s}
     log("computeGrid on the device: %d cells in %.2fs" % (grid.cells, build_s))
-    if not args.no_check:
-        out["map"]["check"] = check_grid(grid, pts, bm, 0.05, rng, args.check_cells)
-        log("grid check:", out["map"]["check"])
     del pts
 
     # sessions

@@ -114,6 +114,22 @@ int a3d_grid_cloud_weight(const a3d_grid* g, const float* x, const float* y, con
                           const float* tx, const float* ty, const float* tz, const float* yaw, size_t n, float roll,
                           float pitch, float* out);
 
+/* ------------------------------------------------------------ VoxelGrid */
+/* pcl::VoxelGrid<pcl::PointXYZ> as the node applies it to each scan before
+ * update (Node.cpp:130-141: setLeafSize(voxel_size) on all axes, PCL
+ * defaults otherwise): one point per occupied voxel of side `leaf`, at the
+ * float centroid of the voxel's finite points, in increasing voxel index
+ * (the order PCL outputs).  points: n points `stride` bytes apart with x, y,
+ * z as their first three floats (12 for packed xyz, 16 for pcl::PointXYZ).
+ * Output: out_x/out_y/out_z with capacity n, *out_n points.  The points of a
+ * voxel are summed in point order; PCL sums them in the order its std::sort
+ * leaves, so a centroid of three or more points may differ in the last bits. */
+int a3d_voxel_grid(const void* points, size_t n, size_t stride, float leaf, float* out_x, float* out_y, float* out_z,
+                   size_t* out_n);
+/* The same on device buffers (synchronous): d_points in, d_x/d_y/d_z out. */
+int a3d_voxel_grid_dev(const void* d_points, size_t n, size_t stride, float leaf, float* d_x, float* d_y, float* d_z,
+                       size_t* out_n);
+
 /* ----------------------------------------------------------- ParticleFilter */
 int a3d_pf_create(a3d_pf** out);
 int a3d_pf_destroy(a3d_pf* pf);

@@ -16,6 +16,8 @@ restates, with the reference's float/double mixing:
 * ``update``               amcl3d/src/ParticleFilter.cpp:121-196
 * ``resample``             amcl3d/src/ParticleFilter.cpp:198-222
 * ``computeRangeWeight``   amcl3d/src/ParticleFilter.cpp:224-244
+* ``voxel_grid``           pcl::VoxelGrid as amcl3d/src/Node.cpp:130-141 calls it
+                           (PCL 1.7.2, not vendored; restated)
 
 ``fp64`` holds an independent double-precision oracle used only to state
 errors.  How each function is pinned (known-answer values from the reference's
@@ -75,6 +77,8 @@ def lib():
         L.orc_resample_indices.argtypes = [_f32p, ctypes.c_int, ctypes.c_float, ctypes.c_int, _u32p]
         L.orc_pf_resample.argtypes = [_f32p, ctypes.c_int, ctypes.c_float, ctypes.c_int, _vp]
         L.orc_slice_int8.argtypes = [_f32p, _sz, _i8p]
+        L.orc_voxel_grid.argtypes = [_f32p, _sz, ctypes.c_float, ctypes.c_int, _f32p]
+        L.orc_voxel_grid.restype = _sz
         _LIB = L
     return _LIB
 
@@ -207,3 +211,13 @@ def slice_int8(prob_plane):
     out = np.empty(pl.shape[0], np.int8)
     lib().orc_slice_int8(pl, pl.shape[0], out)
     return out
+
+
+def voxel_grid(points, leaf, order_mode=0):
+    """pcl::VoxelGrid<PointXYZ> with leaf size `leaf` (Node.cpp:130-141): the
+    centroid of each occupied voxel, in voxel-index order, (k, 3) float32.
+    order_mode 0: PCL's std::sort order inside a voxel; 1: point-index order."""
+    p = _f32(points).reshape(-1, 3)
+    out = np.empty(max(1, p.shape[0]) * 3, np.float32)
+    k = lib().orc_voxel_grid(p.reshape(-1), p.shape[0], float(np.float32(leaf)), int(order_mode), out)
+    return out[:3 * k].reshape(-1, 3)

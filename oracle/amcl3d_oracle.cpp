@@ -32,6 +32,7 @@
 //    indices are 64-bit.
 //  * resample: when the float running sum ends below u (the reference then
 //    reads p_[N], ParticleFilter.cpp:212-216) the index is clamped to N-1.
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -567,6 +568,90 @@ void orc_slice_int8(const float* prob_plane, size_t count, int8_t* out) {
   if (max_prob < 0.000001f) max_prob = 0.000001f;
   max_prob = 100.f / max_prob;
   for (size_t i = 0; i < count; ++i) out[i] = static_cast<int8_t>(prob_plane[i] * max_prob);
+}
+
+// pcl::VoxelGrid<pcl::PointXYZ>::filter as the node calls it (Node.cpp:130-141:
+// setLeafSize(voxel_size) x3, defaults otherwise).  PCL is not vendored in
+// the reference; this restates VoxelGrid::applyFilter of PCL 1.7.2 (the ROS
+// Kinetic / Ubuntu 16.04 package) with its defaults: no filter field,
+// min_points_per_voxel 0, downsample_all_data true (CentroidPoint: float
+// sums of x, y, z over the voxel's points in the order of the sorted index
+// vector, divided by the count as float).  Non-finite points are skipped (the
+// is_dense == false path).  inverse leaf = 1.0f / leaf; bounds and voxel
+// coordinates use float floor:
+//   min_b = (int)floor(min_p * inv); ijk = (int)(floor(p * inv) - (float)min_b)
+//   idx = ijk0 + ijk1 * div0 + ijk2 * div0 * div1   (int)
+// If the voxel count (int64)((max-min)*inv)+1 per axis overflows int32 the
+// input is returned unchanged, as PCL does.  order_mode 0 sorts (idx, point)
+// pairs by idx with std::sort (PCL's unstable sort; the order of points
+// inside a voxel is whatever libstdc++'s introsort leaves), 1 by (idx, point
+// index).  Returns the number of output points written to out_xyz (3 floats
+// each; capacity n).
+size_t orc_voxel_grid(const float* xyz, size_t n, float leaf, int order_mode, float* out_xyz) {
+  const float inv = 1.0f / leaf;
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  size_t nfin = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const float* p = xyz + 3 * i;
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+    ++nfin;
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = std::min(mn[k], p[k]);
+      mx[k] = std::max(mx[k], p[k]);
+    }
+  }
+  if (nfin == 0) return 0;
+  int64_t d[3];
+  for (int k = 0; k < 3; ++k) d[k] = static_cast<int64_t>((mx[k] - mn[k]) * inv) + 1;
+  if (d[0] * d[1] * d[2] > static_cast<int64_t>(INT32_MAX)) {  // indices would overflow: input unchanged
+    std::memcpy(out_xyz, xyz, sizeof(float) * 3 * n);
+    return n;
+  }
+  int min_b[3], max_b[3], div_b[3];
+  for (int k = 0; k < 3; ++k) {
+    min_b[k] = static_cast<int>(std::floor(mn[k] * inv));
+    max_b[k] = static_cast<int>(std::floor(mx[k] * inv));
+    div_b[k] = max_b[k] - min_b[k] + 1;
+  }
+  const int mul[3] = {1, div_b[0], div_b[0] * div_b[1]};
+  struct Entry {
+    unsigned int idx;
+    unsigned int point;
+  };
+  std::vector<Entry> v;
+  v.reserve(nfin);
+  for (size_t i = 0; i < n; ++i) {
+    const float* p = xyz + 3 * i;
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+    int ijk[3];
+    for (int k = 0; k < 3; ++k) ijk[k] = static_cast<int>(std::floor(p[k] * inv) - static_cast<float>(min_b[k]));
+    const int idx = ijk[0] * mul[0] + ijk[1] * mul[1] + ijk[2] * mul[2];
+    v.push_back(Entry{static_cast<unsigned int>(idx), static_cast<unsigned int>(i)});
+  }
+  if (order_mode == 0)
+    std::sort(v.begin(), v.end(), [](const Entry& a, const Entry& b) { return a.idx < b.idx; });
+  else
+    std::sort(v.begin(), v.end(),
+              [](const Entry& a, const Entry& b) { return a.idx < b.idx || (a.idx == b.idx && a.point < b.point); });
+  size_t out = 0;
+  for (size_t first = 0; first < v.size();) {
+    size_t last = first + 1;
+    while (last < v.size() && v[last].idx == v[first].idx) ++last;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (size_t li = first; li < last; ++li) {
+      const float* p = xyz + 3 * (size_t)v[li].point;
+      sx += p[0];
+      sy += p[1];
+      sz += p[2];
+    }
+    const float cnt = static_cast<float>(last - first);
+    out_xyz[3 * out + 0] = sx / cnt;
+    out_xyz[3 * out + 1] = sy / cnt;
+    out_xyz[3 * out + 2] = sz / cnt;
+    ++out;
+    first = last;
+  }
+  return out;
 }
 
 }  // extern "C"

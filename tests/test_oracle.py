@@ -310,3 +310,53 @@ def test_resample_closed_forms():
     p[:, 4] = w[:n] / w[:n].sum()
     q, idx = oracle.pf_resample(p, 0.4)
     assert np.all(q[:, 0] == idx) and np.all(q[:, 4] == F32(1) / F32(n))
+
+
+# ---------------------------------------------------------------- VoxelGrid
+def test_voxel_grid_closed_forms():
+    """pcl::VoxelGrid restated (Node.cpp:130-141): pins from the definition."""
+    leaf = 0.1
+    # one point per voxel: the output is the input, in voxel-index order (x fastest)
+    cells = np.array([[3, 1, 0], [0, 0, 0], [1, 0, 0], [0, 2, 1], [2, 2, 2]])
+    pts = ((cells + 0.5) * leaf).astype(F32)
+    out = oracle.voxel_grid(pts, leaf)
+    key = cells[:, 0] + 4 * (cells[:, 1] + 3 * cells[:, 2])  # div = (4, 3, 3) from the bounds
+    assert np.array_equal(out, pts[np.argsort(key)])
+    # two points in a voxel: their float mean (a sum of two is order-free)
+    p2 = np.array([[0.01, 0.02, 0.03], [0.05, 0.07, 0.09], [0.31, 0.0, 0.0]], F32)
+    out = oracle.voxel_grid(p2, leaf)
+    assert out.shape == (2, 3)
+    assert np.array_equal(out[0], ((p2[0] + p2[1]) / F32(2)).astype(F32))
+    # non-finite points are skipped; an empty or all-NaN cloud gives nothing
+    p3 = np.vstack([p2, [[np.nan, 0, 0], [0, np.inf, 0]]]).astype(F32)
+    assert np.array_equal(oracle.voxel_grid(p3, leaf), out)
+    assert oracle.voxel_grid(np.zeros((0, 3), F32), leaf).shape == (0, 3)
+    assert oracle.voxel_grid(np.full((4, 3), np.nan, F32), leaf).shape == (0, 3)
+    # more voxels than int32 indices: the input comes back unchanged
+    far = np.array([[0, 0, 0], [1000, 1000, 1000], [1, 2, 3]], F32)
+    assert np.array_equal(oracle.voxel_grid(far, 1e-3), far)
+
+
+def test_voxel_grid_against_brute_force():
+    # every voxel's centroid equals the mean of its points up to float
+    # summation order; the voxel set is exactly the set of occupied voxels
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-2.0, 3.0, (20000, 3)).astype(F32)
+    leaf = np.float32(0.25)
+    out = oracle.voxel_grid(pts, leaf)
+    inv = np.float32(1) / leaf
+    ijk = np.floor(pts * inv).astype(np.int64)
+    uniq, inv_idx, counts = np.unique(ijk, axis=0, return_inverse=True, return_counts=True)
+    assert out.shape[0] == uniq.shape[0]
+    mean = np.zeros((uniq.shape[0], 3))
+    np.add.at(mean, inv_idx.reshape(-1), pts.astype(np.float64))
+    mean /= counts[:, None]
+    # output order is increasing voxel index, x fastest
+    lo = ijk.min(0)
+    div = ijk.max(0) - lo + 1
+    key = (uniq[:, 0] - lo[0]) + div[0] * ((uniq[:, 1] - lo[1]) + div[1] * (uniq[:, 2] - lo[2]))
+    assert np.allclose(out, mean[np.argsort(key)], rtol=1e-6, atol=1e-6)
+    # PCL's sort order and point order differ only in the last bits
+    out1 = oracle.voxel_grid(pts, leaf, order_mode=1)
+    ulps = np.abs(out.view(np.int32).astype(np.int64) - out1.view(np.int32))
+    assert ulps.max() <= 8
