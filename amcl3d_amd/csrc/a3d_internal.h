@@ -115,12 +115,12 @@ inline Filters single(const FilterDev& f) {
 }
 
 // Buffers of the particle visiting order (launch_order).
-constexpr uint32_t kOrderKeys = 65536;
+constexpr uint32_t kOrderKeys = 4096;
 struct OrderDev {
   uint32_t* key;    // n
   uint32_t* order;  // n
   uint32_t* hist;   // kOrderKeys + 1
-  float* box;       // 8: min x, y, z, a; then 16 / extent of each
+  float* box;       // 8: lowest level edge of x, y, z, a; then levels per unit of each
 };
 inline size_t order_bytes(uint32_t n) { return (2 * (size_t)n + kOrderKeys + 1) * 4 + 64; }
 inline OrderDev order_view(void* base, uint32_t n) {

@@ -81,11 +81,23 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
     const uint32_t len = min((uint32_t)kCH, n - base);
     if (wave != 0) {
       if (b + 1 < nblk) {
+        // every load of the next block is issued before any is stored, so
+        // the block costs one memory round trip
         const uint32_t nb = base + kCH;
-#pragma unroll 4
-        for (int e = tid - 64; e < K * kCH; e += nthr - 64) {
+        constexpr int kStagers = kChainThreads - 64;
+        constexpr int kR = (K * kCH + kStagers - 1) / kStagers;
+        float t[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int e = tid - 64 + r * kStagers;
           const int k = e / kCH, j = e % kCH;
-          stage[((buf ^ 1) * K + k) * kRow + j] = (nb + j < n) ? src[k][nb + j] : 0.f;
+          t[r] = 0.f;
+          if (e < K * kCH && nb + j < n) t[r] = src[k][nb + j];
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int e = tid - 64 + r * kStagers;
+          if (e < K * kCH) stage[((buf ^ 1) * K + e / kCH) * kRow + e % kCH] = t[r];
         }
       }
       if (PFX >= 0 && b > 0)
@@ -98,12 +110,13 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
       float4* out4 = (PFX >= 0 && lane == PFX) ? prow4 : reinterpret_cast<float4*>(pst + 2 * kCH) + lane;
       const uint32_t ostep = (PFX >= 0 && lane == PFX) ? 1u : 0u;
       const uint32_t n4 = len >> 2;
-      const uint32_t ng = n4 / 16;  // groups of 16 float4, read in two halves
-      // Software pipeline: the loads of the next half are in flight while
-      // the dependent adds of the current half run.
-      auto consume = [&](const float4 (&v)[8], uint32_t q0) {
+      uint32_t q = 0;
+      for (; q + 16 <= n4; q += 16) {
+        float4 v[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 16; ++u) v[u] = row4[q + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
           float4 o;
           acc += v[u].x;
           o.x = acc;
@@ -113,26 +126,9 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
           o.z = acc;
           acc += v[u].w;
           o.w = acc;
-          if (PFX >= 0) out4[(q0 + u) * ostep] = o;
+          if (PFX >= 0) out4[(q + u) * ostep] = o;
         }
-      };
-      float4 va[8], vb[8];
-      if (ng > 0) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) va[u] = row4[u];
       }
-      for (uint32_t gi = 0; gi < ng; ++gi) {
-        const uint32_t q0 = gi * 16;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) vb[u] = row4[q0 + 8 + u];
-        consume(va, q0);
-        if (gi + 1 < ng) {
-#pragma unroll
-          for (int u = 0; u < 8; ++u) va[u] = row4[q0 + 16 + u];
-        }
-        consume(vb, q0 + 8);
-      }
-      const uint32_t q = ng * 16;
       const float* row = reinterpret_cast<const float*>(row4);
       for (uint32_t j = q * 4; j < len; ++j) {
         acc += row[j];

@@ -195,6 +195,12 @@ __device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, 
 // point order (lane shuffles), so it is the reference's sequential float sum.
 // Particles that resampling made near-duplicates sit in neighbouring lanes:
 // one gather then touches few cache lines.
+// Cell of one point under one pose.  With roll = pitch = 0 the terms of the
+// reference's sums that are products with r02 = r12 = r20 = r21 = +-0 are
+// dropped, which is exact for a finite point; a non-finite coordinate makes
+// sz (pz itself) or one of sx, sy (px, py times cos(yaw), which is never 0 for
+// a float yaw) non-finite, so the point fails an axis test, as in the
+// reference.
 template <bool RP0, bool IDX64>
 __device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float py, float pz, const float (&r)[9],
                                               double ox, double oy, double oz, bool act, bool& valid) {
@@ -224,61 +230,67 @@ __device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float 
   return valid ? (uint64_t)(ix + iy * g.size[0] + iz * (uint32_t)g.step_z) : 0ull;
 }
 
+// Points per cloud chunk that a wavefront stages in LDS: one coalesced load
+// per lane and array, then every step reads its point with one LDS read.
+constexpr int kCloudChunk = 64;
+
 template <int Q, bool RP0, bool IDX64>
 __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const float (&r)[9], double ox,
-                                           double oy, double oz, bool act, float& wsum, int& cnt) {
+                                           double oy, double oz, bool act, float& wsum, int& cnt, float4* cl) {
   constexpr int P = 64 / Q;
   constexpr int U = (Q == 1) ? 8 : 4;  // steps in flight
+  constexpr int S = kCloudChunk / Q;   // steps per chunk
+  static_assert(S % U == 0, "a chunk holds whole groups of steps");
   const int lane = lane_id();
   const int ps = lane % P, po = lane / P;
   const uint32_t m = f.m;
-  const float error_z = 0.f;
+  // Each lane counts its own accepted points (an integer count is order-free;
+  // the caller adds the Q lanes of a particle at the end).  The float sum is
+  // not: the Q values of a step reach the particle's lane and are added in
+  // point order.
   auto accumulate = [&](float v, bool val) {
     const float vv = val ? v : 0.0f;  // +0.0f leaves the non-negative sum unchanged
+    cnt += val ? 1 : 0;
     if (Q == 1) {
       wsum += vv;
-      cnt += val ? 1 : 0;
     } else {
-      const uint64_t bm = __ballot(val);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const float vq = (q == 0) ? vv : __shfl(vv, ps + q * P);
-        wsum += vq;
-        cnt += (int)((bm >> (ps + q * P)) & 1ull);
-      }
+      for (int q = 0; q < Q; ++q) wsum += (q == 0) ? vv : __shfl(vv, ps + q * P);
     }
   };
-  uint32_t j0 = 0;
-  for (; j0 + U * Q <= m; j0 += U * Q) {
-    uint64_t idx[U];
-    bool val[U];
+  uint32_t c0 = 0;
+  for (; c0 + kCloudChunk <= m; c0 += kCloudChunk) {
+    wave_lds_sync();  // the previous chunk's reads precede these writes
+    cl[lane] = make_float4(f.cx[c0 + lane], f.cy[c0 + lane], f.cz[c0 + lane], 0.f);
+    wave_lds_sync();
+    for (int s0 = 0; s0 < S; s0 += U) {
+      uint64_t idx[U];
+      bool val[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t j = j0 + u * Q + po;
-      const float px = f.cx[j], py = f.cy[j], pz = f.cz[j] + error_z;
-      const bool fin = !RP0 || (isfinite(px) && isfinite(py) && isfinite(pz));
-      idx[u] = lane_cell<RP0, IDX64>(g, px, py, pz, r, ox, oy, oz, act & fin, val[u]);
+      for (int u = 0; u < U; ++u) {
+        const float4 pt = cl[(s0 + u) * Q + po];
+        idx[u] = lane_cell<RP0, IDX64>(g, pt.x, pt.y, pt.z, r, ox, oy, oz, act, val[u]);
+      }
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) accumulate(v[u], val[u]);
     }
-    float v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
-#pragma unroll
-    for (int u = 0; u < U; ++u) accumulate(v[u], val[u]);
   }
-  for (; j0 < m; j0 += Q) {
-    const uint32_t j = j0 + po;
+  for (; c0 < m; c0 += Q) {  // the last m mod 64 points
+    const uint32_t j = c0 + po;
     const bool pin = j < m;
     const uint32_t js = pin ? j : 0u;
-    const float px = f.cx[js], py = f.cy[js], pz = f.cz[js] + error_z;
-    const bool fin = !RP0 || (isfinite(px) && isfinite(py) && isfinite(pz));
     bool val;
-    const uint64_t idx = lane_cell<RP0, IDX64>(g, px, py, pz, r, ox, oy, oz, act & fin & pin, val);
+    const uint64_t idx = lane_cell<RP0, IDX64>(g, f.cx[js], f.cy[js], f.cz[js], r, ox, oy, oz, act & pin, val);
     accumulate(g.prob[idx], val);
   }
 }
 
 template <int Q, bool IDX64, bool KFILTER>
-__device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out) {
+__device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out,
+                             float4* cl) {
   constexpr int P = 64 / Q;
   const int lane = lane_id();
   const int ps = lane % P, po = lane / P;
@@ -304,9 +316,15 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   float wsum = 0.f;
   int cnt = 0;
   if (f.rp0)
-    lanes_loop<Q, true, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt);
+    lanes_loop<Q, true, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt, cl);
   else
-    lanes_loop<Q, false, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt);
+    lanes_loop<Q, false, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt, cl);
+  if (Q > 1) {
+    int c = cnt;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) c += __shfl(cnt, ps + q * P);
+    cnt = c;
+  }
   if (act && po == 0) {
     const float wp = (cnt <= 10) ? 0.f : wsum / (float)cnt;
     if (KFILTER) {
@@ -321,43 +339,47 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
 template <int Q, bool IDX64>
 __global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
   constexpr uint32_t P = 64 / Q;
+  __shared__ float4 cloud[kWaves * kCloudChunk];
   const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, true>(g, f, p0, min(P, f.n - p0), nullptr);
+  weight_lanes<Q, IDX64, true>(g, f, p0, min(P, f.n - p0), nullptr, cloud + (threadIdx.x >> 6) * kCloudChunk);
 }
 
 template <int Q, bool IDX64>
 __global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  __shared__ float4 cloud[kWaves * kCloudChunk];
   const uint32_t p0 = wave * P;
   if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, false>(g, f, p0, min(P, f.n - p0), out);
+  weight_lanes<Q, IDX64, false>(g, f, p0, min(P, f.n - p0), out, cloud + (threadIdx.x >> 6) * kCloudChunk);
 }
 
 // One work item; a filter that names its own grid (a batch over several
 // maps) is weighted on that grid, else on the launch's.
 template <int Q, bool IDX64>
-__device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it) {
+__device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl) {
   const uint32_t count = min(it.count, (uint32_t)(64 / Q));
   if (f.gdev) {
     const GridDev gf = *f.gdev;
-    weight_lanes<Q, IDX64, true>(gf, f, it.p0, count, nullptr);
+    weight_lanes<Q, IDX64, true>(gf, f, it.p0, count, nullptr, cl);
   } else {
-    weight_lanes<Q, IDX64, true>(g, f, it.p0, count, nullptr);
+    weight_lanes<Q, IDX64, true>(g, f, it.p0, count, nullptr, cl);
   }
 }
 
 template <int Q, bool IDX64>
 __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs, const WorkItem* items,
                                                                uint32_t nitems, uint32_t* counter, int persistent) {
+  __shared__ float4 cloud[kWaves * kCloudChunk];
+  float4* cl = cloud + (threadIdx.x >> 6) * kCloudChunk;
   const int lane = lane_id();
   uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (!persistent) {
     if (k >= nitems) return;
     const WorkItem it = items[k];
-    lanes_item<Q, IDX64>(g, fs[it.filter], it);
+    lanes_item<Q, IDX64>(g, fs[it.filter], it, cl);
     return;
   }
   for (;;) {
@@ -365,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
     if (lane == 0) t = atomicAdd(counter, 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
     if (t >= nitems) break;
-    lanes_item<Q, IDX64>(g, fs[items[t].filter], items[t]);
+    lanes_item<Q, IDX64>(g, fs[items[t].filter], items[t], cl);
   }
 }
 

@@ -114,6 +114,32 @@ def test_grid_prob_only_sparse_map(gpu):
     assert np.array_equal(p.view(np.int32), p2.view(np.int32))
 
 
+def test_grid_file_round_trip(gpu, room, tmp_path):
+    """The .grid cache (Grid3d::saveGrid / loadGrid, Grid3d.cpp:210-275): the
+    file holds u32 sx, sy, sz, f64 sensor_dev, then {f32 dist, f32 prob} per
+    cell, byte for byte what the reference writes for the same cells; loading
+    it gives the same grid; a different sensor_dev is refused."""
+    bm = room["map"]
+    path = tmp_path / "room.grid"
+    room["grid"].save(path)
+    raw = path.read_bytes()
+    sx, sy, sz = np.frombuffer(raw[:12], np.uint32)
+    assert (sx, sy, sz) == (200, 200, 60)
+    assert np.frombuffer(raw[12:20], np.float64)[0] == room["sensor_dev"]
+    cells = np.frombuffer(raw[20:], np.float32).reshape(-1, 2)
+    assert np.array_equal(cells[:, 0], room["odist"].reshape(-1))
+    assert np.array_equal(cells[:, 1].view(np.int32), room["oprob"].reshape(-1).view(np.int32))
+    g = gpu.Grid3d().load(path, bm.min, bm.max, bm.res, room["sensor_dev"])
+    p, d = g.read_cells(dist=True)
+    assert np.array_equal(p, cells[:, 1]) and np.array_equal(d, cells[:, 0])
+    s, parts = room["step"], room["particles"]
+    w1 = room["grid"].cloud_weights(s.cloud, parts[:50, 0], parts[:50, 1], parts[:50, 2], parts[:50, 3])
+    w2 = g.cloud_weights(s.cloud, parts[:50, 0], parts[:50, 1], parts[:50, 2], parts[:50, 3])
+    assert np.array_equal(w1, w2)
+    with pytest.raises(gpu.Amcl3dError):
+        gpu.Grid3d().load(path, bm.min, bm.max, bm.res, room["sensor_dev"] * (1 + 1e-9))
+
+
 # --------------------------------------------------------- computeCloudWeight
 @pytest.fixture(params=[(0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)],
                 ids=lambda mt: "map%d-ppw%d" % mt)

@@ -1,8 +1,9 @@
 """GPU: pcl::VoxelGrid on the device (Node.cpp:130-141) against the oracle.
 
 Bar: the voxel set and order equal the oracle's; centroids bit-exact against
-the oracle summing each voxel's points in point order, and within 8 ulps of
-PCL's own order (std::sort), in which ~5% of centroids of 3+ points differ."""
+the oracle summing each voxel's points in point order, and within 2e-6
+relative of PCL's own order (std::sort), in which centroids of 3+ points
+can differ in the last bits (5% of coordinates at a 0.1 m leaf, 27% at 0.37 m)."""
 import numpy as np
 import pytest
 
@@ -30,8 +31,10 @@ def test_voxel_grid_matches_oracle(gpu, leaf):
     assert np.array_equal(got.view(np.int32), ref1.view(np.int32))
     assert got.shape == ref0.shape
     ulps = np.abs(got.view(np.int32).astype(np.int64) - ref0.view(np.int32))
-    print("  vs PCL order: %.4f of coordinates differ, max %d ulp" % ((ulps != 0).mean(), ulps.max()), flush=True)
-    assert ulps.max() <= 8
+    rel = np.abs(got.astype(np.float64) - ref0) / np.maximum(np.abs(ref0), leaf)
+    print("  vs PCL order: %.4f of coordinates differ, max %d ulp, max rel %.2g"
+          % ((ulps != 0).mean(), ulps.max(), rel.max()), flush=True)
+    assert rel.max() <= 2e-6  # summation order of a voxel's points
 
 
 def test_voxel_grid_pointxyz_stride_and_edges(gpu):
