@@ -117,18 +117,18 @@ inline Filters single(const FilterDev& f) {
 // Buffers of the particle visiting order (launch_order).
 constexpr uint32_t kOrderKeys = 4096;
 struct OrderDev {
+  double* part;     // 64 x 8 partial sums (moments of x, y, z, a)
   uint32_t* key;    // n
   uint32_t* order;  // n
   uint32_t* hist;   // kOrderKeys + 1
-  float* box;       // 8: lowest level edge of x, y, z, a; then levels per unit of each
 };
-inline size_t order_bytes(uint32_t n) { return (2 * (size_t)n + kOrderKeys + 1) * 4 + 64; }
+inline size_t order_bytes(uint32_t n) { return 64 * 8 * 8 + (2 * (size_t)n + kOrderKeys + 1) * 4; }
 inline OrderDev order_view(void* base, uint32_t n) {
   OrderDev o;
-  o.key = static_cast<uint32_t*>(base);
+  o.part = static_cast<double*>(base);
+  o.key = reinterpret_cast<uint32_t*>(o.part + 64 * 8);
   o.order = o.key + n;
   o.hist = o.order + n;
-  o.box = reinterpret_cast<float*>(o.hist + kOrderKeys + 1);
   return o;
 }
 
@@ -180,7 +180,8 @@ int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 //   [0, 4n)    update tail: in-map wp, wr, then the weighted x, y, z, a
 //   [4n, 10n)  resample: gathered x, y, z, a, wp, wr
 //   [10n, 12n) resample FP64 prefix (n doubles, 8-byte aligned)
-//   [12n, 13n) resample float running sum c
+//   [12n, 13n) resample float running sum c: its value at the end of every
+//              group of 64 weights (ceil(n / 64) floats)
 //   then FP64 block totals, then 16 floats of per-filter sums (wtp, wtr, wt)
 __host__ __device__ inline size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
 __host__ __device__ inline size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }

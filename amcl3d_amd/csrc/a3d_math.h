@@ -63,6 +63,7 @@ __device__ inline bool axis_cell(float s, double off, float ext_hi, double res, 
 // The same axis test without branches, for the common case: the cell from the
 // double reciprocal, and `near` set where the product lies within 1e-9 of an
 // integer.  The caller re-derives such axes with axis_exact (rare).
+// ix is meaningful only where ok holds (0 <= k < size); callers select on ok.
 __device__ inline void axis_fast(float s, double off, float ext_hi, double inv_res, uint32_t size, bool& ok,
                                  bool& near, uint32_t& ix, float& q) {
   q = (float)((double)s + off);
@@ -71,13 +72,13 @@ __device__ inline void axis_fast(float s, double off, float ext_hi, double inv_r
   const double fr = d - k;
   near = (fr < 1e-9) | (fr > 1.0 - 1e-9);
   ok = (q >= 0.0f) & (q < ext_hi) & (k < (double)size);
-  ix = (uint32_t)fmin(fmax(k, 0.0), 4294967295.0);
+  ix = ok ? (uint32_t)k : 0u;
 }
 
 __device__ inline void axis_exact(float q, float ext_hi, double res, uint32_t size, bool& ok, uint32_t& ix) {
   const double k = floor((double)q / res);
   ok = (q >= 0.0f) & (q < ext_hi) & (k < (double)size);
-  ix = (uint32_t)fmin(fmax(k, 0.0), 4294967295.0);
+  ix = ok ? (uint32_t)k : 0u;
 }
 
 // Grid3d::isIntoMap, Grid3d.cpp:201-208 (float vs double compares).

@@ -53,7 +53,10 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
   g.inv_res = 1.0 / res;
   g.step_z = (uint64_t)g.size[0] * g.size[1];
   g.cells = g.step_z * g.size[2];
-  g.idx64 = g.cells > 0xffffffffull ? 1u : 0u;
+  // 32-bit cell indices use 24-bit multiply-adds: (iy + sy * iz) * sx + ix
+  // needs sx < 2^24 and sy * sz < 2^24; other grids take the 64-bit path.
+  g.idx64 = (g.cells > 0xffffffffull || g.size[0] >= (1u << 24) ||
+             (uint64_t)g.size[1] * g.size[2] >= (1ull << 24)) ? 1u : 0u;
   // Test knob: A3D_FORCE_IDX64=1 takes the 64-bit index path on any map.
   const char* force = std::getenv("A3D_FORCE_IDX64");
   if (force && force[0] == '1') g.idx64 = 1u;

@@ -13,7 +13,7 @@
 // grid-wide elementwise kernels (grid y = filter).
 //
 // update:   tail_a  -> chain(wtp, wtr) -> tail_c -> chain(wt) -> tail_e
-//           -> chain(mean x, y, z, a and resample's c)
+//           -> chain(mean x, y, z, a) beside chain(resample's c)
 // resample: [chain(c) unless update just produced it] -> select -> commit
 // init:     init_a -> chain(wt) -> init_c -> chain(mean)
 #include "a3d_math.h"
@@ -25,12 +25,12 @@ constexpr int kChainThreads = 1024;
 constexpr int kCH = 2048;  // elements per staged block
 constexpr int kRow = kCH + 12;  // LDS row stride: the rows of up to 5 chains start 12 banks apart
 constexpr int kEw = 256;   // elementwise block size
+constexpr uint32_t kGroup = 64;  // weights per running-sum checkpoint
 
 // which chains a chain_kernel launch runs
 enum ChainWhich {
   kTailTotals = 0,   // wtp, wtr over the in-map wp, wr     -> sums[0], sums[1]
   kWeightTotal = 1,  // wt over w                           -> sums[2]
-  kTailMean = 2,     // mean x, y, z, a (+ c over w)        -> mean[0..3], c
   kPrefixOnly = 3,   // c over w                            -> c
   kInitMean = 4      // mean x, y, z, a                     -> mean[0..3]
 };
@@ -38,10 +38,93 @@ enum ChainWhich {
 __device__ inline const FilterDev& fsel(const Filters& fs, uint32_t i) { return fs.arr ? fs.arr[i] : fs.one; }
 __device__ inline float* sums_of(const FilterDev& f) { return f.scratch + sc_sums(f.n); }
 
+// LDS of one chain workgroup: two staging rows per chain.
+template <int K, int PFX>
+struct ChainLds {
+  static constexpr int kFloats = 2 * K * kRow;
+};
+
+// Sequential float sums src[k][0..n) -> *dst[k] for k < K (lane k of wave 0
+// runs chain k); with PFX >= 0, chain PFX's running sum at the end of every
+// group of kGroup elements goes to prefix[0 .. ceil(n / kGroup)).
+template <int K, int PFX>
+__device__ void chain_body(uint32_t n, const float* const (&src)[K], float* const (&dst)[K], float* prefix,
+                           float* lds) {
+  float* stage = lds;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, nthr = blockDim.x;
+  const uint32_t nblk = (n + kCH - 1) / kCH;
+  float acc = 0.f;  // every reference sum starts from 0 (0 + x is x exactly)
+  for (int e = tid; e < K * kCH; e += nthr) {
+    const int k = e / kCH, j = e % kCH;
+    stage[k * kRow + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
+  }
+  __syncthreads();
+  for (uint32_t b = 0; b < nblk; ++b) {
+    const int buf = b & 1;
+    const uint32_t base = b * kCH;
+    const uint32_t len = min((uint32_t)kCH, n - base);
+    // Waves are dealt to the CU's four SIMDs round-robin: waves 4, 8, 12
+    // share wave 0's SIMD and stay idle, so the chain's dependent adds never
+    // wait for another wave's instructions.  The other 12 waves stage.
+    if ((wave & 3) != 0) {
+      const int st = ((wave >> 2) * 3 + (wave & 3) - 1) * 64 + lane;  // 0 .. kStagers-1
+      constexpr int kStagers = (kChainThreads / 64 / 4) * 3 * 64;
+      if (b + 1 < nblk) {
+        // every load of the next block is issued before any is stored, so
+        // the block costs one memory round trip
+        const uint32_t nb = base + kCH;
+        constexpr int kR = (K * kCH + kStagers - 1) / kStagers;
+        float t[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int e = st + r * kStagers;
+          const int k = e / kCH, j = e % kCH;
+          t[r] = 0.f;
+          if (e < K * kCH && nb + j < n) t[r] = src[k][nb + j];
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const int e = st + r * kStagers;
+          if (e < K * kCH) stage[((buf ^ 1) * K + e / kCH) * kRow + e % kCH] = t[r];
+        }
+      }
+    } else if (wave == 0 && lane < K) {
+      const float4* row4 = reinterpret_cast<const float4*>(stage + (buf * K + lane) * kRow);
+      // The prefix lane keeps one checkpoint per 64 elements, the running
+      // sum at the group's end (a store per element would stall the chain);
+      // resample re-adds a group's weights from the previous checkpoint,
+      // which reproduces every running sum bit for bit.
+      const uint32_t n4 = len >> 2;
+      uint32_t q = 0;
+      for (; q + 16 <= n4; q += 16) {
+        float4 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = row4[q + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          acc += v[u].x;
+          acc += v[u].y;
+          acc += v[u].z;
+          acc += v[u].w;
+        }
+        if (PFX >= 0 && lane == PFX) prefix[(base + q * 4) / kGroup] = acc;
+      }
+      const float* row = reinterpret_cast<const float*>(row4);
+      for (uint32_t j = q * 4; j < len; ++j) acc += row[j];
+      if (PFX >= 0 && lane == PFX && q * 4 < len) prefix[(base + len - 1) / kGroup] = acc;  // last, partial group
+    }
+    __syncthreads();
+  }
+  if (wave == 0 && lane < K) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (lane == k) *dst[k] = acc;
+  }
+}
+
 template <int K, int PFX>
 __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int which) {
-  __shared__ float4 stage4[2 * K * kRow / 4];
-  __shared__ float4 pstage4[PFX >= 0 ? 2 * kCH / 4 + 8 : 1];  // + one junk float4 per chain lane
+  __shared__ float4 lds4[ChainLds<K, PFX>::kFloats / 4];
   const FilterDev& f = fsel(fs, blockIdx.x);
   if (which == kPrefixOnly && f.no_resample) return;
   const uint32_t n = f.n;
@@ -57,95 +140,33 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
   } else if (which == kWeightTotal || which == kPrefixOnly) {
     src[0] = f.w;
     dst[0] = (which == kWeightTotal) ? sums_of(f) + 2 : sums_of(f) + 3;
-  } else {  // kTailMean, kInitMean
+  } else {  // kInitMean
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      src[k] = (k < 4) ? sc + (size_t)k * n : f.w;
-      dst[k] = (k < 4) ? f.mean + k : sums_of(f) + 3;
+      src[k] = sc + (size_t)k * n;
+      dst[k] = f.mean + k;
     }
   }
   if (PFX >= 0) prefix = sc + sc_c32(n);
-  float* stage = reinterpret_cast<float*>(stage4);
-  float* pst = reinterpret_cast<float*>(pstage4);
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, nthr = blockDim.x;
-  const uint32_t nblk = (n + kCH - 1) / kCH;
-  float acc = 0.f;  // every reference sum starts from 0 (0 + x is x exactly)
-  for (int e = tid; e < K * kCH; e += nthr) {
-    const int k = e / kCH, j = e % kCH;
-    stage[k * kRow + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
-  }
-  __syncthreads();
-  for (uint32_t b = 0; b < nblk; ++b) {
-    const int buf = b & 1;
-    const uint32_t base = b * kCH;
-    const uint32_t len = min((uint32_t)kCH, n - base);
-    if (wave != 0) {
-      if (b + 1 < nblk) {
-        // every load of the next block is issued before any is stored, so
-        // the block costs one memory round trip
-        const uint32_t nb = base + kCH;
-        constexpr int kStagers = kChainThreads - 64;
-        constexpr int kR = (K * kCH + kStagers - 1) / kStagers;
-        float t[kR];
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          const int e = tid - 64 + r * kStagers;
-          const int k = e / kCH, j = e % kCH;
-          t[r] = 0.f;
-          if (e < K * kCH && nb + j < n) t[r] = src[k][nb + j];
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          const int e = tid - 64 + r * kStagers;
-          if (e < K * kCH) stage[((buf ^ 1) * K + e / kCH) * kRow + e % kCH] = t[r];
-        }
-      }
-      if (PFX >= 0 && b > 0)
-        for (int j = tid - 64; j < kCH; j += nthr - 64) prefix[base - kCH + j] = pst[(buf ^ 1) * kCH + j];
-    } else if (lane < K) {
-      const float4* row4 = reinterpret_cast<const float4*>(stage + (buf * K + lane) * kRow);
-      // the prefix lane writes its running sums; the other chain lanes write
-      // to a junk slot of their own, so the store needs no branch
-      float4* prow4 = reinterpret_cast<float4*>(pst + buf * kCH);
-      float4* out4 = (PFX >= 0 && lane == PFX) ? prow4 : reinterpret_cast<float4*>(pst + 2 * kCH) + lane;
-      const uint32_t ostep = (PFX >= 0 && lane == PFX) ? 1u : 0u;
-      const uint32_t n4 = len >> 2;
-      uint32_t q = 0;
-      for (; q + 16 <= n4; q += 16) {
-        float4 v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = row4[q + u];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          float4 o;
-          acc += v[u].x;
-          o.x = acc;
-          acc += v[u].y;
-          o.y = acc;
-          acc += v[u].z;
-          o.z = acc;
-          acc += v[u].w;
-          o.w = acc;
-          if (PFX >= 0) out4[(q + u) * ostep] = o;
-        }
-      }
-      const float* row = reinterpret_cast<const float*>(row4);
-      for (uint32_t j = q * 4; j < len; ++j) {
-        acc += row[j];
-        if (PFX >= 0 && lane == PFX) pst[buf * kCH + j] = acc;
-      }
-    }
-    __syncthreads();
-  }
-  if (PFX >= 0 && nblk > 0) {
-    const uint32_t lb = (nblk - 1) * kCH;
-    const int buf = (nblk - 1) & 1;
-    for (uint32_t j = tid; j < (uint32_t)kCH && lb + j < n; j += nthr) prefix[lb + j] = pst[buf * kCH + j];
-  }
-  if (wave == 0 && lane < K) {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (lane == k) *dst[k] = acc;
+  chain_body<K, PFX>(n, src, dst, prefix, reinterpret_cast<float*>(lds4));
+}
+
+// The last chains of update: the four mean terms (workgroup row 0) and
+// resample's running sum c over the new weights (row 1) run side by side.
+__global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) {
+  __shared__ float4 lds4[ChainLds<4, -1>::kFloats / 4];
+  static_assert(ChainLds<1, 0>::kFloats <= ChainLds<4, -1>::kFloats, "LDS of the prefix chain");
+  const FilterDev& f = fsel(fs, blockIdx.x);
+  const uint32_t n = f.n;
+  float* sc = f.scratch;
+  if (blockIdx.y == 0) {
+    const float* const src[4] = {sc, sc + n, sc + 2 * (size_t)n, sc + 3 * (size_t)n};
+    float* const dst[4] = {f.mean, f.mean + 1, f.mean + 2, f.mean + 3};
+    chain_body<4, -1>(n, src, dst, nullptr, reinterpret_cast<float*>(lds4));
+  } else {
+    const float* const src[1] = {f.w};
+    float* const dst[1] = {sums_of(f) + 3};
+    chain_body<1, 0>(n, src, dst, sc + sc_c32(n), reinterpret_cast<float*>(lds4));
   }
 }
 
@@ -316,7 +337,32 @@ __global__ void resample_select_kernel(Filters fs, int mode) {
     const float factor = 1.f / (float)n;  // 1.f / p_.size() (size_t -> float)
     const float r = factor * u01;
     const float u = r + factor * (float)m;
-    i = first_not_below(sc + sc_c32(n), n, u);
+    // the group holding the first running sum c_i with !(u > c_i), from the
+    // checkpoints; then c_i itself, re-added in the reference's order
+    const float* cp = sc + sc_c32(n);
+    const uint32_t ng = (n + kGroup - 1) / kGroup;
+    uint32_t gi = 0, hi = ng;
+    while (gi < hi) {
+      const uint32_t mid = gi + ((hi - gi) >> 1);
+      if (!(u > cp[mid]))
+        hi = mid;
+      else
+        gi = mid + 1;
+    }
+    if (gi == ng) {
+      i = n - 1;  // the float total ends below u: clamped (the reference reads p_[n])
+    } else {
+      float c = gi ? cp[gi - 1] : 0.0f;  // 0 + w_0 is w_0 exactly
+      const uint32_t e = min(n, (gi + 1) * kGroup);
+      i = e - 1;
+      for (uint32_t k = gi * kGroup; k < e; ++k) {
+        c += f.w[k];
+        if (!(u > c)) {
+          i = k;
+          break;
+        }
+      }
+    }
   } else {
     const double* c64 = reinterpret_cast<const double*>(sc + sc_c64(n));
     const double u = ((double)u01 + (double)m) * (1.0 / (double)n);
@@ -382,8 +428,8 @@ void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
   tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
   chain_kernel<1, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
   tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
-  // the mean and, in lane 4, resample's running sum c over the new weights
-  chain_kernel<5, 4><<<fs.count, kChainThreads, 0, s>>>(fs, kTailMean);
+  // the mean and, in a second workgroup per filter, resample's running sum c
+  mean_prefix_kernel<<<dim3(fs.count, 2), kChainThreads, 0, s>>>(fs);
 }
 
 void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s) {

@@ -13,8 +13,9 @@
 // deviation around the particles' mean (so the levels stay populated however
 // concentrated the set is).  Four kernels, no library calls and no memset,
 // so the sequence can be captured into a HIP graph:
-//   moments  one workgroup: mean and deviation of x, y, z, a; clears the histogram
-//   keys     key per particle, histogram by atomics
+//   moments  64 blocks: partial sums of x, y, z, a and their squares; clears the histogram
+//   keys     every block derives the levels from the partial sums; key per
+//            particle, histogram per block in LDS, added to the global one
 //   scan     one workgroup: exclusive scan of the 4,096 counts
 //   scatter  order[offset[key]++] = i (order inside a key is arbitrary)
 #include "a3d_math.h"
@@ -34,29 +35,26 @@ __device__ inline uint32_t level8(float v, float lo, float scale) {
   return (uint32_t)t;
 }
 
-__global__ __launch_bounds__(kBT) void order_moments_kernel(FilterDev f, OrderDev o) {
-  __shared__ double red[8][kBT / 64];
+constexpr int kMB = 64;   // blocks of the moments kernel
+constexpr int kKT = 256;  // threads per block of the keys and scatter kernels
+
+// Partial sums of x, y, z, a and their squares (in double) per block, and the
+// histogram cleared for the keys kernel.
+__global__ __launch_bounds__(kKT) void order_moments_kernel(FilterDev f, OrderDev o) {
+  __shared__ double red[8][kKT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (uint32_t k = tid; k < kOrderKeys + 1; k += kBT) o.hist[k] = 0u;
+  const uint32_t gt = blockIdx.x * kKT + tid, gs = kMB * kKT;
+  for (uint32_t k = gt; k < kOrderKeys + 1; k += gs) o.hist[k] = 0u;
   double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
   const float* arr[4] = {f.x, f.y, f.z, f.a};
-  constexpr int kU = 8;  // loads in flight per thread
-  for (uint32_t i0 = tid; i0 < f.n; i0 += kBT * kU) {
-    float v[kU][4];
+  for (uint32_t i = gt; i < f.n; i += gs) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const uint32_t i = i0 + (uint32_t)u * kBT;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v[u][k] = (i < f.n) ? arr[k][i] : 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      const float v = arr[k][i];
+      const double d = isfinite(v) ? (double)v : 0.0;
+      s1[k] += d;
+      s2[k] += d * d;
     }
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double d = isfinite(v[u][k]) ? (double)v[u][k] : 0.0;
-        s1[k] += d;
-        s2[k] += d * d;
-      }
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -70,30 +68,52 @@ __global__ __launch_bounds__(kBT) void order_moments_kernel(FilterDev f, OrderDe
     }
   }
   __syncthreads();
-  if (tid < 4) {
+  if (tid < 8) {
+    double a = 0.0;
+    for (int w = 0; w < kKT / 64; ++w) a += red[tid][w];
+    o.part[blockIdx.x * 8 + tid] = a;
+  }
+}
+
+// The levels of one block: mean - 2 sd and 2 / sd per coordinate.
+__device__ inline void order_levels(const FilterDev& f, const OrderDev& o, float* box) {
+  if (threadIdx.x < 4) {
     double a = 0.0, b = 0.0;
-    for (int w = 0; w < kBT / 64; ++w) {
-      a += red[tid][w];
-      b += red[4 + tid][w];
+    for (int p = 0; p < kMB; ++p) {
+      a += o.part[p * 8 + threadIdx.x];
+      b += o.part[p * 8 + 4 + threadIdx.x];
     }
     const double n = (double)(f.n ? f.n : 1u);
     const double mean = a / n;
     const double var = b / n - mean * mean;
     const double sd = var > 0.0 ? sqrt(var) : 0.0;
-    o.box[tid] = (float)(mean - 2.0 * sd);
-    o.box[4 + tid] = (sd > 0.0 && sd < 1e30) ? (float)(2.0 / sd) : 0.0f;
+    box[threadIdx.x] = (float)(mean - 2.0 * sd);
+    box[4 + threadIdx.x] = (sd > 0.0 && sd < 1e30) ? (float)(2.0 / sd) : 0.0f;
   }
 }
 
-__global__ void order_keys_kernel(FilterDev f, OrderDev o) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n) return;
-  const uint32_t key = spread3(level8(f.x[i], o.box[0], o.box[4])) |
-                       (spread3(level8(f.y[i], o.box[1], o.box[5])) << 1) |
-                       (spread3(level8(f.z[i], o.box[2], o.box[6])) << 2) |
-                       (spread3(level8(f.a[i], o.box[3], o.box[7])) << 3);
-  o.key[i] = key;
-  atomicAdd(&o.hist[key], 1u);
+__device__ inline uint32_t order_key(const FilterDev& f, uint32_t i, const float* box) {
+  return spread3(level8(f.x[i], box[0], box[4])) | (spread3(level8(f.y[i], box[1], box[5])) << 1) |
+         (spread3(level8(f.z[i], box[2], box[6])) << 2) | (spread3(level8(f.a[i], box[3], box[7])) << 3);
+}
+
+// Keys, counted per block in LDS and added to the global histogram once per
+// occupied key (few global atomics however concentrated the particles are).
+__global__ __launch_bounds__(kKT) void order_keys_kernel(FilterDev f, OrderDev o) {
+  __shared__ uint32_t h[kOrderKeys];
+  __shared__ float box[8];
+  for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT) h[k] = 0u;
+  order_levels(f, o, box);
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kKT + threadIdx.x;
+  if (i < f.n) {
+    const uint32_t key = order_key(f, i, box);
+    o.key[i] = key;
+    atomicAdd(&h[key], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT)
+    if (h[k]) atomicAdd(&o.hist[k], h[k]);
 }
 
 __global__ __launch_bounds__(kBT) void order_scan_kernel(OrderDev o) {
@@ -120,22 +140,34 @@ __global__ __launch_bounds__(kBT) void order_scan_kernel(OrderDev o) {
   }
 }
 
-__global__ void order_scatter_kernel(FilterDev f, OrderDev o) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n) return;
-  const uint32_t pos = atomicAdd(&o.hist[o.key[i]], 1u);
-  o.order[pos] = i;  // pos < n: the offsets of key k cover exactly its count
+// order[offset[key]++] = i: a block reserves one range per occupied key and
+// places its particles of that key inside it (in arbitrary order).
+__global__ __launch_bounds__(kKT) void order_scatter_kernel(FilterDev f, OrderDev o) {
+  __shared__ uint32_t cnt[kOrderKeys];
+  for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT) cnt[k] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kKT + threadIdx.x;
+  uint32_t key = 0, rank = 0;
+  if (i < f.n) {
+    key = o.key[i];
+    rank = atomicAdd(&cnt[key], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT)
+    if (cnt[k]) cnt[k] = atomicAdd(&o.hist[k], cnt[k]);  // now the block's base for key k
+  __syncthreads();
+  if (i < f.n) o.order[cnt[key] + rank] = i;  // < n: the ranges of key k cover exactly its count
 }
 
 }  // namespace
 
 void launch_order(const FilterDev& f, const OrderDev& o, hipStream_t s) {
   if (f.n == 0) return;
-  const uint32_t blocks = (f.n + 255) / 256;
-  order_moments_kernel<<<1, kBT, 0, s>>>(f, o);
-  order_keys_kernel<<<blocks, 256, 0, s>>>(f, o);
+  const uint32_t blocks = (f.n + kKT - 1) / kKT;
+  order_moments_kernel<<<kMB, kKT, 0, s>>>(f, o);
+  order_keys_kernel<<<blocks, kKT, 0, s>>>(f, o);
   order_scan_kernel<<<1, kBT, 0, s>>>(o);
-  order_scatter_kernel<<<blocks, 256, 0, s>>>(f, o);
+  order_scatter_kernel<<<blocks, kKT, 0, s>>>(f, o);
 }
 
 }  // namespace a3d

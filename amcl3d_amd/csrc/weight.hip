@@ -227,7 +227,10 @@ __device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float 
   }
   valid = act & okx & oky & okz;
   if (IDX64) return valid ? (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z : 0ull;
-  return valid ? (uint64_t)(ix + iy * g.size[0] + iz * (uint32_t)g.step_z) : 0ull;
+  // 24-bit multiply-adds: the host takes the 64-bit path unless sx < 2^24
+  // and sy * sz < 2^24 (ix, iy, iz are below their sizes, or 0)
+  const uint32_t lin = __umul24(__umul24(iz, g.size[1]) + iy, g.size[0]) + ix;
+  return valid ? (uint64_t)lin : 0ull;
 }
 
 // Points per cloud chunk that a wavefront stages in LDS: one coalesced load
@@ -266,11 +269,12 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     for (int s0 = 0; s0 < S; s0 += U) {
       uint64_t idx[U];
       bool val[U];
+      float4 pt[U];  // all LDS reads of the group first
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const float4 pt = cl[(s0 + u) * Q + po];
-        idx[u] = lane_cell<RP0, IDX64>(g, pt.x, pt.y, pt.z, r, ox, oy, oz, act, val[u]);
-      }
+      for (int u = 0; u < U; ++u) pt[u] = cl[(s0 + u) * Q + po];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        idx[u] = lane_cell<RP0, IDX64>(g, pt[u].x, pt[u].y, pt[u].z, r, ox, oy, oz, act, val[u]);
       float v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
