@@ -114,8 +114,8 @@ def _timed(fn):
     return time.perf_counter() - t0
 
 
-def served(amcl3d_amd, W, maps, nsessions=256, rate_hz=10.0, duration_s=8.0, seed=5, max_inflight=2,
-           max_batch_evals=1_500_000_000, latest_only=True, poses_per_map=12, n_range=(1000, 50000),
+def served(amcl3d_amd, W, maps, nsessions=256, rate_hz=10.0, duration_s=8.0, seed=5, max_inflight=3,
+           max_batch_evals=800_000_000, latest_only=True, poses_per_map=12, n_range=(1000, 50000),
            m_range=(2000, 20000)):
     """Sessions sending Poisson update requests at rate_hz each.
 
