@@ -60,10 +60,20 @@ __device__ inline bool axis_cell(float s, double off, float ext_hi, double res, 
   return true;
 }
 
+// double -> int32 with saturation (v_cvt_i32_f64: values out of range clamp
+// to INT32_MIN / INT32_MAX, NaN gives 0), defined for every input.
+__device__ inline int cvt_i32_sat(double k) {
+  int r;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(k));
+  return r;
+}
+
 // The same axis test without branches, for the common case: the cell from the
 // double reciprocal, and `near` set where the product lies within 1e-9 of an
 // integer.  The caller re-derives such axes with axis_exact (rare).
-// ix is meaningful only where ok holds (0 <= k < size); callers select on ok.
+// The range test 0 <= k < size is one unsigned compare of the saturated
+// integer (q >= 0 <=> k >= 0 for a non-NaN q, and a NaN q fails q < ext_hi;
+// sizes are below 2^31).  ix is meaningful only where ok holds.
 __device__ inline void axis_fast(float s, double off, float ext_hi, double inv_res, uint32_t size, bool& ok,
                                  bool& near, uint32_t& ix, float& q) {
   q = (float)((double)s + off);
@@ -71,8 +81,8 @@ __device__ inline void axis_fast(float s, double off, float ext_hi, double inv_r
   const double k = floor(d);
   const double fr = d - k;
   near = (fr < 1e-9) | (fr > 1.0 - 1e-9);
-  ok = (q >= 0.0f) & (q < ext_hi) & (k < (double)size);
-  ix = ok ? (uint32_t)k : 0u;
+  ix = (uint32_t)cvt_i32_sat(k);
+  ok = (q < ext_hi) & (ix < size);
 }
 
 __device__ inline void axis_exact(float q, float ext_hi, double res, uint32_t size, bool& ok, uint32_t& ix) {

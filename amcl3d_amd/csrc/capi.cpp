@@ -272,6 +272,9 @@ static int grid_alloc(const double mn[3], const double mx[3], double res, double
     if (!(std::isfinite(mn[a]) && std::isfinite(mx[a]) && mx[a] > mn[a]))
       return fail(A3D_ERR_ARG, "grid: bounds must be finite with max > min");
   if (!(std::isfinite(sensor_dev) && sensor_dev > 0)) return fail(A3D_ERR_ARG, "grid: sensor_dev must be positive");
+  for (int a = 0; a < 3; ++a)  // cell coordinates are compared as int32 on the device
+    if (!(std::ceil((mx[a] - mn[a]) / res) < 2147483648.0))
+      return fail(A3D_ERR_ARG, "grid: 2^31 or more cells along an axis");
   a3d_grid* G = new a3d_grid();
   grid_meta(mn, mx, res, G->g);
   G->sensor_dev = sensor_dev;
