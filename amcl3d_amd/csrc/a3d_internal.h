@@ -3,8 +3,9 @@
 // Device layout (HBM):
 //  * grid: prob as one float per cell, x fastest, index ix + iy*sx + iz*sx*sy
 //    (the reference's cell order, Grid3d.cpp:187), 64-bit when the map has
-//    2^32 cells or more; the squared distance field is a second, optional,
-//    array of the same shape.
+//    2^32 - 1 cells or more, followed by one float 0 that rejected points
+//    read; the squared distance field is a second, optional, array of the
+//    cells' shape.
 //  * filter: structure of arrays x, y, z, a, w, wp, wr (one float each per
 //    particle, the fields of struct Particle, ParticleFilter.h:35-49).
 //  * cloud: three float arrays x, y, z in cloud order.
@@ -34,8 +35,8 @@ bool hip_ok(hipError_t e, const char* what);
 
 // ------------------------------------------------------------ device views
 struct GridDev {
-  const float* prob;
-  float* dist;  // optional (map build output)
+  const float* prob;  // cells + 1 floats: prob[cells] = 0 is the cell of a rejected point
+  float* dist;        // optional (map build output)
   uint32_t size[3];
   uint32_t idx64;
   uint64_t step_z;  // sx*sy
@@ -47,6 +48,16 @@ struct GridDev {
   float pad_;
   double res;
   double inv_res;
+  // For a float q other than -0.0: q >= 0 && q < ext_hi && floor(q/res) < size
+  // <=> bits(q) < lim (both bounds are monotone in q, so they fold into one
+  // float bound, and a negative q or a NaN has a larger bit pattern).
+  uint32_t lim[3];
+  uint32_t pad2_;
+  // Half-width of the window around an integer inside which q*inv_res may
+  // floor differently from the correctly rounded q/res (max(1e-9, size*2^-48)),
+  // and twice that.
+  double near_w[3];
+  double near_w2[3];
 };
 
 // One filter and the parameters of the call being made on it.  Arrays are

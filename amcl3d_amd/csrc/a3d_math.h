@@ -43,49 +43,52 @@ __device__ inline void rotation9(float roll, float pitch, float yaw, float r[9])
 //   q >= 0 && q < ext  (in double)                          Grid3d.cpp:178-179
 //   ix = (uint32)floor((double)q / res), ix < size          Grid3d.cpp:181-185
 // The division is replaced by a multiplication with the double reciprocal;
-// whenever the product lies within 1e-9 of an integer (where the two could
-// round to different sides) the correctly rounded division is evaluated
-// instead, so ix always equals the reference's.
-__device__ inline bool axis_cell(float s, double off, float ext_hi, double res, double inv_res, uint32_t size,
-                                 uint32_t& ix) {
+// whenever the product lies within w (GridDev::near_w) of an integer, where
+// the two could floor differently, the correctly rounded division is
+// evaluated instead, so ix always equals the reference's.
+__device__ inline bool axis_cell(float s, double off, float ext_hi, double res, double inv_res, double w,
+                                 uint32_t size, uint32_t& ix) {
   const float q = (float)((double)s + off);
   if (!(q >= 0.0f && q < ext_hi)) return false;
   const double qd = (double)q;
   const double d = qd * inv_res;
   double k = floor(d);
   const double fr = d - k;
-  if (fr < 1e-9 || fr > 1.0 - 1e-9) k = floor(qd / res);
+  if (fr < w || fr > 1.0 - w) k = floor(qd / res);
   if (!(k < (double)size)) return false;
   ix = (uint32_t)k;
   return true;
 }
 
-// double -> int32 with saturation (v_cvt_i32_f64: values out of range clamp
-// to INT32_MIN / INT32_MAX, NaN gives 0), defined for every input.
+// double -> int32 rounding toward zero with saturation (v_cvt_i32_f64: values
+// out of range clamp to INT32_MIN / INT32_MAX, NaN gives 0): defined for
+// every input, unlike a C conversion.
 __device__ inline int cvt_i32_sat(double k) {
   int r;
   asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(k));
   return r;
 }
 
-// The same axis test without branches, for the common case: the cell from the
-// double reciprocal, and `near` set where the product lies within 1e-9 of an
-// integer.  The caller re-derives such axes with axis_exact (rare).
-// The range test 0 <= k < size is one unsigned compare of the saturated
-// integer (q >= 0 <=> k >= 0 for a non-NaN q, and a NaN q fails q < ext_hi;
-// sizes are below 2^31).  ix is meaningful only where ok holds.
-__device__ inline void axis_fast(float s, double off, float ext_hi, double inv_res, uint32_t size, bool& ok,
-                                 bool& near, uint32_t& ix, float& q) {
-  q = (float)((double)s + off);
+// The same axis without branches, for a q that is not -0.0 (the caller
+// routes the lanes where q could be -0.0 to axis_exact):
+//   in   = bits(q) < lim, i.e. 0 <= q < ext && floor(q/res) < size, exactly;
+//   ix   = trunc(q * inv_res), which is floor for the q >= 0 that pass;
+//   near = q * inv_res within w of an integer.
+// Away from the window, |q*inv_res - q/res| <= size * 2^-51 < w / 8 keeps
+// both on the same side of every integer, so ix is the reference's index;
+// the caller recomputes the near lanes with axis_exact.
+__device__ inline void axis_fast(float s, double off, uint32_t lim, double inv_res, double w, double w2, bool& in,
+                                 bool& near, uint32_t& ix) {
+  const float q = (float)((double)s + off);
   const double d = (double)q * inv_res;
-  const double k = floor(d);
-  const double fr = d - k;
-  near = (fr < 1e-9) | (fr > 1.0 - 1e-9);
-  ix = (uint32_t)cvt_i32_sat(k);
-  ok = (q < ext_hi) & (ix < size);
+  ix = (uint32_t)cvt_i32_sat(d);
+  near = __builtin_amdgcn_fract(d + w) < w2;  // w2 = 2w
+  in = __float_as_uint(q) < lim;
 }
 
-__device__ inline void axis_exact(float q, float ext_hi, double res, uint32_t size, bool& ok, uint32_t& ix) {
+__device__ inline void axis_exact(float s, double off, float ext_hi, double res, uint32_t size, bool& ok,
+                                  uint32_t& ix) {
+  const float q = (float)((double)s + off);
   const double k = floor((double)q / res);
   ok = (q >= 0.0f) & (q < ext_hi) & (k < (double)size);
   ix = ok ? (uint32_t)k : 0u;

@@ -51,11 +51,34 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
   }
   g.res = res;
   g.inv_res = 1.0 / res;
+  for (int a = 0; a < 3; ++a) {
+    // t = the smallest non-negative float with floor((double)t / res) >= size
+    // (bit patterns of non-negative floats are ordered like their values)
+    auto past = [&](uint32_t b) {
+      float f;
+      std::memcpy(&f, &b, 4);
+      return std::floor((double)f / res) >= (double)g.size[a];
+    };
+    uint32_t lo = 0, hi = 0x7f800000u;  // +inf is past
+    if (past(0)) hi = 0;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      (past(mid) ? hi : lo) = mid;
+    }
+    uint32_t eb;
+    std::memcpy(&eb, &g.ext_hi[a], 4);
+    g.lim[a] = std::min(eb, hi);
+    g.near_w[a] = std::max(1e-9, std::ldexp((double)g.size[a], -48));
+    g.near_w2[a] = 2.0 * g.near_w[a];
+  }
+  g.pad2_ = 0;
   g.step_z = (uint64_t)g.size[0] * g.size[1];
   g.cells = g.step_z * g.size[2];
   // 32-bit cell indices use 24-bit multiply-adds: (iy + sy * iz) * sx + ix
-  // needs sx < 2^24 and sy * sz < 2^24; other grids take the 64-bit path.
-  g.idx64 = (g.cells > 0xffffffffull || g.size[0] >= (1u << 24) ||
+  // needs sx < 2^24 and sy * sz < 2^24, and the zero cell after the last
+  // (index cells) must fit a 32-bit record count; other grids take the
+  // 64-bit path.
+  g.idx64 = (g.cells >= 0xffffffffull || g.size[0] >= (1u << 24) ||
              (uint64_t)g.size[1] * g.size[2] >= (1ull << 24)) ? 1u : 0u;
   // Test knob: A3D_FORCE_IDX64=1 takes the 64-bit index path on any map.
   const char* force = std::getenv("A3D_FORCE_IDX64");
@@ -279,9 +302,15 @@ static int grid_alloc(const double mn[3], const double mx[3], double res, double
   grid_meta(mn, mx, res, G->g);
   G->sensor_dev = sensor_dev;
   const size_t bytes = G->g.cells * sizeof(float);
-  if (hipMalloc(&G->prob, bytes) != hipSuccess) {
+  // one more float, 0, after the last cell: the cell that rejected points read
+  if (hipMalloc(&G->prob, bytes + sizeof(float)) != hipSuccess) {
     delete G;
     return fail(A3D_ERR_NOMEM, "grid: cannot allocate " + std::to_string(bytes) + " bytes of prob");
+  }
+  if (hipMemset(G->prob + G->g.cells, 0, sizeof(float)) != hipSuccess) {
+    (void)hipFree(G->prob);
+    delete G;
+    return fail(A3D_ERR_HIP, "grid: cannot clear the zero cell");
   }
   if (keep_dist && hipMalloc(&G->dist, bytes) != hipSuccess) {
     (void)hipFree(G->prob);

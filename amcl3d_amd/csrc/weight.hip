@@ -7,11 +7,11 @@
 // caller picks one with a3d_set_mapping (an autotuner times both).
 //
 // Mapping 0, particle lanes (lanes_*_kernel).  A wavefront carries P = 64/Q
-// particles; lane l holds particle l mod P and, per step, point j0 + l div P,
+// particles; lane l holds particle l div Q and, per step, point j0 + l mod Q,
 // so the wavefront advances Q consecutive points of the cloud per step and
-// the P lanes of one point gather for P nearby poses.  The Q values of one
-// particle reach its lane by shuffles and are added in point order, so the
-// sum is the reference's sequential float sum.
+// gathers them for P nearby poses.  The Q values of one particle reach its
+// first lane by quad permutes and are added in point order, so the sum is
+// the reference's sequential float sum.
 //
 // Mapping 1, point lanes (weight_*_kernel).  One wavefront owns a block of up
 // to PB particles of one filter and walks the whole cloud in chunks of 64
@@ -22,9 +22,14 @@
 // the cloud.  The 64 values land in row i of a per-wave LDS tile (0 for a
 // point outside the map).  Lane i then adds its row in point order, so each
 // particle's sum is the reference's sequential float sum over the cloud,
-// bit for bit (adding +0.0f for a rejected point leaves a non-negative sum
+// bit for bit (a rejected point adds -0.0f, which leaves every sum
 // unchanged).  The count of accepted points comes from a ballot.
 #include "a3d_math.h"
+
+// The structured buffer load (buffer_load_dword ... idxen), bound to the
+// compiler's intrinsic by name.
+__device__ float a3d_struct_load_f32(__amdgpu_buffer_rsrc_t r, int vindex, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.struct.ptr.buffer.load.f32");
 
 namespace a3d {
 namespace {
@@ -62,9 +67,9 @@ __device__ __forceinline__ uint64_t cell_of(const GridDev& g, float px, float py
   }
   uint32_t ix = 0, iy = 0, iz = 0;
   bool ok = pv;
-  ok = axis_cell(sx, ox, g.ext_hi[0], g.res, g.inv_res, g.size[0], ix) && ok;
-  ok = axis_cell(sy, oy, g.ext_hi[1], g.res, g.inv_res, g.size[1], iy) && ok;
-  ok = axis_cell(sz, oz, g.ext_hi[2], g.res, g.inv_res, g.size[2], iz) && ok;
+  ok = axis_cell(sx, ox, g.ext_hi[0], g.res, g.inv_res, g.near_w[0], g.size[0], ix) && ok;
+  ok = axis_cell(sy, oy, g.ext_hi[1], g.res, g.inv_res, g.near_w[1], g.size[1], iy) && ok;
+  ok = axis_cell(sz, oz, g.ext_hi[2], g.res, g.inv_res, g.near_w[2], g.size[2], iz) && ok;
   valid = ok;
   if (!ok) return 0;
   if (IDX64) return (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
@@ -109,7 +114,7 @@ __device__ __forceinline__ void eval_chunk(const GridDev& g, float px, float py,
     for (int u = 0; u < U; ++u) {
       const uint32_t i = i0 + u;
       if (i < count && ((active >> i) & 1ull)) {
-        tile[i * kTS + lane] = val[u] ? v[u] : 0.f;
+        tile[i * kTS + lane] = val[u] ? v[u] : -0.0f;  // x + -0.0f == x for every x
         const uint64_t vm = __ballot(val[u]);
         cnt += (lane == (int)i) ? (int)__popcll(vm) : 0;
       }
@@ -190,21 +195,62 @@ __device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, 
 
 // ---------------------------------------------------------------------------
 // Mapping 0, lanes = particles.  A wavefront carries P = 64/Q particles; lane
-// l holds particle l % P and, at each step, point (step*Q + l / P) of the
-// cloud.  A particle's sum takes the Q values of a step from its Q lanes in
-// point order (lane shuffles), so it is the reference's sequential float sum.
-// Particles that resampling made near-duplicates sit in neighbouring lanes:
-// one gather then touches few cache lines.
-// Cell of one point under one pose.  With roll = pitch = 0 the terms of the
-// reference's sums that are products with r02 = r12 = r20 = r21 = +-0 are
-// dropped, which is exact for a finite point; a non-finite coordinate makes
-// sz (pz itself) or one of sx, sy (px, py times cos(yaw), which is never 0 for
-// a float yaw) non-finite, so the point fails an axis test, as in the
-// reference.
-template <bool RP0, bool IDX64>
-__device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float py, float pz, const float (&r)[9],
-                                              double ox, double oy, double oz, bool act, bool& valid) {
-  float sx, sy, sz;
+// l holds particle l / Q and, at each step, point (step*Q + l % Q) of the
+// cloud, so the Q lanes of a particle are consecutive lanes of one quad.  A
+// particle's sum takes the Q values of a step from its quad in point order
+// (DPP quad permutes folded into the adds), so it is the reference's
+// sequential float sum.  Particles that resampling made near-duplicates sit
+// in neighbouring lanes: one gather then touches few cache lines.
+
+// Fourth descriptor word of a buffer resource on the gfx9 family: 32-bit data
+// format, no swizzle, no per-lane stride add.
+constexpr int kBufWord3 = 0x00020000;
+
+// prob[i] for a cell index of the map: 32-bit indices load through a
+// structured buffer of 4-byte records (no 64-bit address arithmetic per lane),
+// 64-bit ones from the global pointer.  Index `cells` is the zero cell.
+template <bool IDX64>
+struct ProbSrc;
+
+template <>
+struct ProbSrc<false> {
+  using Idx = uint32_t;
+  __amdgpu_buffer_rsrc_t r;
+  Idx zero;
+  __device__ explicit ProbSrc(const GridDev& g)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.prob, (short)4, (int)(uint32_t)(g.cells + 1), kBufWord3)),
+        zero((uint32_t)g.cells) {}
+  __device__ float operator()(Idx i) const { return a3d_struct_load_f32(r, (int)i, 0, 0, 0); }
+};
+
+template <>
+struct ProbSrc<true> {
+  using Idx = uint64_t;
+  const float* p;
+  Idx zero;
+  __device__ explicit ProbSrc(const GridDev& g) : p(g.prob), zero(g.cells) {}
+  __device__ float operator()(Idx i) const { return p[i]; }
+};
+
+// The pose of a lane's particle.  A lane without an active particle has
+// NaN offsets, so every point fails its axis tests (q is NaN).  slow marks
+// offsets within 2^-96 of zero, the only ones for which q can round to -0.0
+// (the fast axis test needs a q that is not -0.0); such lanes take the exact
+// path at every point.
+struct LanePose {
+  float r[9];
+  double ox, oy, oz;
+  bool slow;
+};
+
+// With roll = pitch = 0 the terms of the reference's sums that are products
+// with r02 = r12 = r20 = r21 = +-0 are dropped, which is exact for a finite
+// point; a non-finite coordinate makes sz (pz itself) or one of sx, sy (px,
+// py times cos(yaw), which is never 0 for a float yaw) non-finite, so the
+// point fails an axis test, as in the reference.
+template <bool RP0>
+__device__ __forceinline__ void rotate(const float (&r)[9], float px, float py, float pz, float& sx, float& sy,
+                                       float& sz) {
   if (RP0) {
     sx = px * r[0] + py * r[1];
     sy = px * r[3] + py * r[4];
@@ -214,23 +260,67 @@ __device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float 
     sy = px * r[3] + py * r[4] + pz * r[5];
     sz = px * r[6] + py * r[7] + pz * r[8];
   }
-  bool okx, oky, okz, nx, ny, nz;
-  uint32_t ix, iy, iz;
-  float qx, qy, qz;
-  axis_fast(sx, ox, g.ext_hi[0], g.inv_res, g.size[0], okx, nx, ix, qx);
-  axis_fast(sy, oy, g.ext_hi[1], g.inv_res, g.size[1], oky, ny, iy, qy);
-  axis_fast(sz, oz, g.ext_hi[2], g.inv_res, g.size[2], okz, nz, iz, qz);
-  if (__builtin_expect(nx | ny | nz, 0)) {
-    if (nx) axis_exact(qx, g.ext_hi[0], g.res, g.size[0], okx, ix);
-    if (ny) axis_exact(qy, g.ext_hi[1], g.res, g.size[1], oky, iy);
-    if (nz) axis_exact(qz, g.ext_hi[2], g.res, g.size[2], okz, iz);
+}
+
+template <bool IDX64>
+__device__ __forceinline__ typename ProbSrc<IDX64>::Idx linear(const GridDev& g, uint32_t ix, uint32_t iy,
+                                                               uint32_t iz) {
+  if constexpr (IDX64) {
+    return (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
+  } else {
+    // (iz * sy + iy) * sx + ix in two 24-bit multiply-adds: the host takes
+    // the 64-bit path unless sx < 2^24 and sy * sz < 2^24 (exact for in-map
+    // cells; other lanes are replaced by the zero cell)
+    uint32_t a, b;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"(iz), "s"(g.size[1]), "v"(iy));
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(b) : "v"(a), "s"(g.size[0]), "v"(ix));
+    return b;
   }
-  valid = act & okx & oky & okz;
-  if (IDX64) return valid ? (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z : 0ull;
-  // 24-bit multiply-adds: the host takes the 64-bit path unless sx < 2^24
-  // and sy * sz < 2^24 (ix, iy, iz are below their sizes, or 0)
-  const uint32_t lin = __umul24(__umul24(iz, g.size[1]) + iy, g.size[0]) + ix;
-  return valid ? (uint64_t)lin : 0ull;
+}
+
+// Cell of one point under the lane's pose, fast form: valid (in the map),
+// near (some axis needs the exact division, or the lane is slow) and the
+// index to load (the zero cell when not valid).
+template <bool RP0, bool IDX64>
+__device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev& g, const ProbSrc<IDX64>& src,
+                                                                  const LanePose& lp, float px, float py, float pz,
+                                                                  bool& valid, bool& near) {
+  float sx, sy, sz;
+  rotate<RP0>(lp.r, px, py, pz, sx, sy, sz);
+  bool inx, iny, inz, nx, ny, nz;
+  uint32_t ix, iy, iz;
+  axis_fast(sx, lp.ox, g.lim[0], g.inv_res, g.near_w[0], g.near_w2[0], inx, nx, ix);
+  axis_fast(sy, lp.oy, g.lim[1], g.inv_res, g.near_w[1], g.near_w2[1], iny, ny, iy);
+  axis_fast(sz, lp.oz, g.lim[2], g.inv_res, g.near_w[2], g.near_w2[2], inz, nz, iz);
+  valid = inx & iny & inz;
+  near = nx | ny | nz | lp.slow;
+  const auto lin = linear<IDX64>(g, ix, iy, iz);
+  return valid ? lin : src.zero;
+}
+
+// The same with the correctly rounded division on every axis.
+template <bool RP0, bool IDX64>
+__device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_exact(const GridDev& g, const ProbSrc<IDX64>& src,
+                                                                   const LanePose& lp, float px, float py, float pz,
+                                                                   bool& valid) {
+  float sx, sy, sz;
+  rotate<RP0>(lp.r, px, py, pz, sx, sy, sz);
+  bool okx, oky, okz;
+  uint32_t ix, iy, iz;
+  axis_exact(sx, lp.ox, g.ext_hi[0], g.res, g.size[0], okx, ix);
+  axis_exact(sy, lp.oy, g.ext_hi[1], g.res, g.size[1], oky, iy);
+  axis_exact(sz, lp.oz, g.ext_hi[2], g.res, g.size[2], okz, iz);
+  valid = okx & oky & okz;
+  const auto lin = linear<IDX64>(g, ix, iy, iz);
+  return valid ? lin : src.zero;
+}
+
+// Value of member K of this lane's group of Q consecutive lanes (a quad
+// permute: Q = 4 broadcasts lane K of the quad, Q = 2 lane K of the pair).
+template <int Q, int K>
+__device__ __forceinline__ float group_member(float v) {
+  constexpr int ctrl = (Q == 4) ? K * 0x55 : (K | (K << 2) | ((K + 2) << 4) | ((K + 2) << 6));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false));
 }
 
 // Points per cloud chunk that a wavefront stages in LDS: one coalesced load
@@ -238,27 +328,32 @@ __device__ __forceinline__ uint64_t lane_cell(const GridDev& g, float px, float 
 constexpr int kCloudChunk = 64;
 
 template <int Q, bool RP0, bool IDX64>
-__device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const float (&r)[9], double ox,
-                                           double oy, double oz, bool act, float& wsum, int& cnt, float4* cl) {
-  constexpr int P = 64 / Q;
+__device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
+                                           int& cnt, float4* cl) {
+  using Idx = typename ProbSrc<IDX64>::Idx;
   constexpr int U = (Q == 1) ? 8 : 4;  // steps in flight
   constexpr int S = kCloudChunk / Q;   // steps per chunk
   static_assert(S % U == 0, "a chunk holds whole groups of steps");
+  const ProbSrc<IDX64> src(g);
   const int lane = lane_id();
-  const int ps = lane % P, po = lane / P;
+  const int po = lane % Q;
   const uint32_t m = f.m;
   // Each lane counts its own accepted points (an integer count is order-free;
   // the caller adds the Q lanes of a particle at the end).  The float sum is
-  // not: the Q values of a step reach the particle's lane and are added in
-  // point order.
-  auto accumulate = [&](float v, bool val) {
-    const float vv = val ? v : 0.0f;  // +0.0f leaves the non-negative sum unchanged
-    cnt += val ? 1 : 0;
-    if (Q == 1) {
-      wsum += vv;
+  // not: the Q values of a step reach the particle's first lane and are added
+  // in point order.  A rejected point reads the zero cell, -0.0f, which leaves
+  // every sum unchanged (x + -0.0f == x, also for x = -0.0f).
+  auto accumulate = [&](float v) {
+    if constexpr (Q == 1) {
+      wsum += v;
+    } else if constexpr (Q == 2) {
+      wsum += group_member<2, 0>(v);
+      wsum += group_member<2, 1>(v);
     } else {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) wsum += (q == 0) ? vv : __shfl(vv, ps + q * P);
+      wsum += group_member<4, 0>(v);
+      wsum += group_member<4, 1>(v);
+      wsum += group_member<4, 2>(v);
+      wsum += group_member<4, 3>(v);
     }
   };
   uint32_t c0 = 0;
@@ -267,37 +362,66 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     cl[lane] = make_float4(f.cx[c0 + lane], f.cy[c0 + lane], f.cz[c0 + lane], 0.f);
     wave_lds_sync();
     for (int s0 = 0; s0 < S; s0 += U) {
-      uint64_t idx[U];
-      bool val[U];
+      Idx idx[U];
+      bool nr[U];
       float4 pt[U];  // all LDS reads of the group first
 #pragma unroll
       for (int u = 0; u < U; ++u) pt[u] = cl[(s0 + u) * Q + po];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        idx[u] = lane_cell<RP0, IDX64>(g, pt[u].x, pt[u].y, pt[u].z, r, ox, oy, oz, act, val[u]);
+      for (int u = 0; u < U; ++u) {
+        bool val;
+        idx[u] = cell_fast<RP0, IDX64>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
+        cnt += val ? 1 : 0;
+      }
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) any |= nr[u];
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
+        // rare: redo the flagged lanes with the division, re-reading their
+        // points (nothing of the fast form is kept for this path)
+        const volatile float* vcl = reinterpret_cast<const volatile float*>(cl);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (nr[u]) {
+            const int k = 4 * ((s0 + u) * Q + po);
+            bool ve;
+            const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, vcl[k], vcl[k + 1], vcl[k + 2], ve);
+            cnt += (ve ? 1 : 0) - (idx[u] != src.zero ? 1 : 0);
+            idx[u] = ie;
+          }
+        }
+      }
       float v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = g.prob[idx[u]];
+      for (int u = 0; u < U; ++u) v[u] = src(idx[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) accumulate(v[u], val[u]);
+      for (int u = 0; u < U; ++u) accumulate(v[u]);
     }
   }
   for (; c0 < m; c0 += Q) {  // the last m mod 64 points
     const uint32_t j = c0 + po;
     const bool pin = j < m;
     const uint32_t js = pin ? j : 0u;
-    bool val;
-    const uint64_t idx = lane_cell<RP0, IDX64>(g, f.cx[js], f.cy[js], f.cz[js], r, ox, oy, oz, act & pin, val);
-    accumulate(g.prob[idx], val);
+    // a lane past the cloud's end takes a NaN point, which every test rejects
+    const float px = pin ? f.cx[js] : __builtin_nanf(""), py = f.cy[js], pz = f.cz[js];
+    bool val, nr;
+    Idx idx = cell_fast<RP0, IDX64>(g, src, lp, px, py, pz, val, nr);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0, 0)) {
+      bool ve;
+      const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, px, py, pz, ve);
+      idx = nr ? ie : idx;
+      val = nr ? ve : val;
+    }
+    cnt += val ? 1 : 0;
+    accumulate(src(idx));
   }
 }
 
 template <int Q, bool IDX64, bool KFILTER>
 __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out,
                              float4* cl) {
-  constexpr int P = 64 / Q;
   const int lane = lane_id();
-  const int ps = lane % P, po = lane / P;
+  const int ps = lane / Q, po = lane % Q;
   const bool has = (uint32_t)ps < count;
   const uint32_t pk = p0 + (has ? (uint32_t)ps : 0u);
   const uint32_t pi = f.order ? f.order[pk] : pk;
@@ -310,23 +434,26 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   }
   const bool act = KFILTER ? (has && into_map(g, tx, ty, tz)) : has;
   if (__ballot(act) == 0) return;
-  float r[9];
-  rotation9(f.roll, f.pitch, ta, r);
-  const double ox = (double)tx - g.min[0];
-  const double oy = (double)ty - g.min[1];
-  const double oz = (double)tz - g.min[2];
+  LanePose lp;
+  rotation9(f.roll, f.pitch, ta, lp.r);
+  const double nan = __builtin_nan("");
+  lp.ox = act ? (double)tx - g.min[0] : nan;
+  lp.oy = act ? (double)ty - g.min[1] : nan;
+  lp.oz = act ? (double)tz - g.min[2] : nan;
+  constexpr double kTiny = 0x1p-96;
+  lp.slow = fabs(lp.ox) < kTiny || fabs(lp.oy) < kTiny || fabs(lp.oz) < kTiny;
   float wr = 0.f;
   if (KFILTER && act && po == 0) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
   float wsum = 0.f;
   int cnt = 0;
   if (f.rp0)
-    lanes_loop<Q, true, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt, cl);
+    lanes_loop<Q, true, IDX64>(g, f, lp, wsum, cnt, cl);
   else
-    lanes_loop<Q, false, IDX64>(g, f, r, ox, oy, oz, act, wsum, cnt, cl);
+    lanes_loop<Q, false, IDX64>(g, f, lp, wsum, cnt, cl);
   if (Q > 1) {
     int c = cnt;
 #pragma unroll
-    for (int q = 1; q < Q; ++q) c += __shfl(cnt, ps + q * P);
+    for (int q = 1; q < Q; ++q) c += __shfl(cnt, ps * Q + q);
     cnt = c;
   }
   if (act && po == 0) {
