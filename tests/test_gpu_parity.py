@@ -211,6 +211,87 @@ def test_cloud_weights_64bit_index_path(gpu, room, shape, monkeypatch):
     assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
+@pytest.fixture(scope="module")
+def long_axis(gpu):
+    """A map 15.9 million cells long in x (one cell in y and z) whose resolution
+    r = RN(1120.125 / 1354172) makes q / r land on an integer for q = k * 1120.125
+    while q * RN(1/r) rounds just below it: the cells where a product with the
+    reciprocal floors one lower than the reference's division.  Cell values are
+    signed (a map loaded from cells may hold any float)."""
+    res = float(np.float64(F32(1120.125)) / 1354172)
+    mx = (15_900_000 * res, res, res)
+    size = oracle.grid_sizes((0.0, 0.0, 0.0), mx, res)
+    assert size[0] < (1 << 24) and size[1] == 1 and size[2] == 1
+    prob = np.random.default_rng(21).normal(size=int(size[0])).astype(F32)
+    q = (np.arange(1, 12, dtype=np.float64) * 1120.125).astype(F32)
+    d_fast, d_exact = np.floor(q.astype(np.float64) * (1.0 / res)), np.floor(q.astype(np.float64) / res)
+    # the product floors lower beyond 5e6, where a fixed 1e-9 window around
+    # integers would no longer cover the rounding of q * (1/r)
+    assert np.any((d_fast != d_exact) & (d_exact > 5e6))
+    rng = np.random.default_rng(22)
+    rand = rng.uniform(2.0, mx[0] - 2.0, 53).astype(F32)
+    qx = np.concatenate([q, rand, q[::-1]])
+    # pose (1, r/2, r/2), yaw 0: sx = px, so q = (float)(px + 1.0)
+    cloud = np.zeros((qx.shape[0], 3), F32)
+    cloud[:, 0] = qx - F32(1.0)
+    assert np.array_equal((cloud[:, 0].astype(np.float64) + 1.0).astype(F32), qx)
+    g = gpu.Grid3d().from_cells(prob, (0.0, 0.0, 0.0), mx, res, 0.05)
+    return dict(res=res, mx=mx, prob=prob, cloud=cloud, grid=g,
+                ogrid=oracle.GridMeta((0.0, 0.0, 0.0), mx, res, prob))
+
+
+def _long_axis_poses(res, n=96):
+    tx = np.full(n, 1.0, F32)
+    tx[n // 2:] = np.random.default_rng(23).uniform(0.5, 2.0, n - n // 2).astype(F32)
+    t = np.full(n, F32(res / 2), F32)
+    return tx, t, t.copy(), np.zeros(n, F32)
+
+
+def test_cloud_weights_long_axis_near_integer_cells(gpu, long_axis, shape):
+    la = long_axis
+    tx, ty, tz, ya = _long_axis_poses(la["res"])
+    w = la["grid"].cloud_weights(la["cloud"], tx, ty, tz, ya)
+    ow = oracle.cloud_weight_batch(la["ogrid"], la["cloud"], tx, ty, tz, ya)
+    assert np.all(ow[:48] != 0)
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
+def test_cloud_weights_long_axis_64bit_index_path(gpu, long_axis, monkeypatch):
+    monkeypatch.setenv("A3D_FORCE_IDX64", "1")
+    la = long_axis
+    g = gpu.Grid3d().from_cells(la["prob"], (0.0, 0.0, 0.0), la["mx"], la["res"], 0.05)
+    tx, ty, tz, ya = _long_axis_poses(la["res"])
+    w = g.cloud_weights(la["cloud"], tx, ty, tz, ya)
+    ow = oracle.cloud_weight_batch(la["ogrid"], la["cloud"], tx, ty, tz, ya)
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
+@pytest.mark.parametrize("mn", [0.0, 1e-50])
+def test_cloud_weights_zero_offsets(gpu, shape, mn):
+    """Poses at (or within 1e-45 of) the map's minimum corner: the offsets
+    t - min are 0, -0 or tiny, and a point at the origin gives q = -0.0 or a
+    tiny negative double that rounds to -0.0f, which the reference accepts
+    (q >= 0) into cell 0."""
+    res = 0.25
+    mnv = (mn, mn, mn)
+    mx = (8.0, 8.0, 8.0)
+    size = oracle.grid_sizes(mnv, mx, res)
+    prob = np.random.default_rng(24).uniform(0.0, 1.0, int(np.prod(size))).astype(F32)
+    g = gpu.Grid3d().from_cells(prob, mnv, mx, res, 0.05)
+    og = oracle.GridMeta(mnv, mx, res, prob)
+    rng = np.random.default_rng(25)
+    base = rng.uniform(0.0, 7.9, (40, 3)).astype(F32)
+    special = np.array([[0.0, 0.0, 0.0], [-0.0, -0.0, -0.0], [-0.0, 0.0, -0.0], [1e-45, -1e-45, 0.0],
+                        [-1e-45, 0.0, 0.0], [0.0, 0.125, 0.0], [-1e-30, 0.0, 0.0]], F32)
+    cloud = np.concatenate([special, base, special])
+    t = np.array([0.0, -0.0, 1e-45, -1e-45, 0.5, 2.0], F32)
+    tx, ty, tz = (a.reshape(-1) for a in np.meshgrid(t, t, t, indexing="ij"))
+    ya = np.zeros(tx.shape[0], F32)
+    w = g.cloud_weights(cloud, tx, ty, tz, ya)
+    ow = oracle.cloud_weight_batch(og, cloud, tx, ty, tz, ya)
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
 # ---------------------------------------------------------------- the filter
 def test_init_bit_exact(gpu):
     rng = np.random.default_rng(2)
