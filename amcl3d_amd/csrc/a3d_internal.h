@@ -52,7 +52,9 @@ struct GridDev {
   // <=> bits(q) < lim (both bounds are monotone in q, so they fold into one
   // float bound, and a negative q or a NaN has a larger bit pattern).
   uint32_t lim[3];
-  uint32_t pad2_;
+  // 1 when, on every axis, trunc(q * inv_res) equals floor(q / res) for every
+  // float q in [0, lim): the cell index needs no window test at all.
+  uint32_t clean;
   // Half-width of the window around an integer inside which q*inv_res may
   // floor differently from the correctly rounded q/res (max(1e-9, size*2^-48)),
   // and twice that.

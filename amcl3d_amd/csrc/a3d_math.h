@@ -86,6 +86,18 @@ __device__ inline void axis_fast(float s, double off, uint32_t lim, double inv_r
   in = __float_as_uint(q) < lim;
 }
 
+// axis_fast on a clean grid (GridDev::clean: the truncated product is the
+// reference's index for every q in range), with no window test.  ZERO also
+// accepts q = -0.0 (for lanes whose offset is within 2^-96 of zero), which
+// lies in cell 0 whenever the axis has a cell (lim > 0).
+template <bool ZERO>
+__device__ inline void axis_clean(float s, double off, uint32_t lim, double inv_res, bool& in, uint32_t& ix) {
+  const float q = (float)((double)s + off);
+  ix = (uint32_t)cvt_i32_sat((double)q * inv_res);
+  in = __float_as_uint(q) < lim;
+  if (ZERO) in = in | ((q == 0.0f) & (lim != 0u));
+}
+
 __device__ inline void axis_exact(float s, double off, float ext_hi, double res, uint32_t size, bool& ok,
                                   uint32_t& ix) {
   const float q = (float)((double)s + off);

@@ -38,6 +38,28 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
+// Whether the device's fast cell index along one axis, trunc(q * inv_res),
+// equals the reference's floor(q / res) (correctly rounded division) for
+// every float q >= 0 whose cell is below size.  The two products are within
+// 2^-52 relative of q/res, so they can straddle an integer n only when
+// |q - n*res| <= q * 2^-51.9, far below the spacing of floats around q: q is
+// then one of the floats bracketing n*res, which are among the float nearest
+// to the double n*res and its two neighbours.  Checking those three floats for
+// n = 1..size covers every q; longer axes are not checked (reported unclean).
+static bool axis_clean(double res, double inv_res, uint32_t size) {
+  if (size > (1u << 24) || !(res > 1e-30)) return false;
+  for (uint32_t n = 1; n <= size; ++n) {
+    const float p = (float)((double)n * res);
+    const float c[3] = {std::nextafter(p, -INFINITY), p, std::nextafter(p, INFINITY)};
+    for (float q : c) {
+      if (!(q >= 0.0f)) continue;
+      const double qd = (double)q;
+      if (std::trunc(qd * inv_res) != std::floor(qd / res)) return false;
+    }
+  }
+  return true;
+}
+
 // Grid metadata exactly as PointCloudTools.cpp:93-101 and Grid3d.cpp:151-153.
 void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
   for (int a = 0; a < 3; ++a) {
@@ -71,7 +93,11 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
     g.near_w[a] = std::max(1e-9, std::ldexp((double)g.size[a], -48));
     g.near_w2[a] = 2.0 * g.near_w[a];
   }
-  g.pad2_ = 0;
+  g.clean = (axis_clean(res, g.inv_res, g.size[0]) && axis_clean(res, g.inv_res, g.size[1]) &&
+             axis_clean(res, g.inv_res, g.size[2])) ? 1u : 0u;
+  // Test knob: A3D_FORCE_NEAR_TEST=1 keeps the window test on any map.
+  const char* near_test = std::getenv("A3D_FORCE_NEAR_TEST");
+  if (near_test && near_test[0] == '1') g.clean = 0u;
   g.step_z = (uint64_t)g.size[0] * g.size[1];
   g.cells = g.step_z * g.size[2];
   // 32-bit cell indices use 24-bit multiply-adds: (iy + sy * iz) * sx + ix

@@ -290,7 +290,14 @@ bool a3d_server::launch(Slot& s) {
   fs.max_n = max_n;
   launch_draws(fs, s.d_seedstep, s.d_seedstep + nf, s.d_u01, st);
   launch_predict(fs, st);
-  launch_weight(maps[map0]->g, fs, s.d_items, (uint32_t)s.h_items.size(), s.d_counter, ppw, 1, st);
+  // the launch's grid: 64-bit indices if any map of the batch needs them,
+  // clean only if every map is
+  GridDev gl = maps[map0]->g;
+  for (auto& r : s.reqs) {
+    gl.idx64 |= maps[sessions[r.session].map]->g.idx64;
+    gl.clean &= maps[sessions[r.session].map]->g.clean;
+  }
+  launch_weight(gl, fs, s.d_items, (uint32_t)s.h_items.size(), s.d_counter, ppw, 1, st);
   launch_update_tail(maps[map0]->g, fs, st);
   // means to the host before resample (getMean is read after update, Node.cpp:167)
   launch_gather_means(fs, s.d_means, st);

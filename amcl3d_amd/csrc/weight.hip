@@ -278,22 +278,41 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx linear(const GridDev& g,
   }
 }
 
+// How the fast cell index is made exact:
+//   kWindow     window test on every axis, flagged lanes (and slow lanes)
+//               redone with the division;
+//   kClean      a clean grid (the truncated product is always the
+//               reference's index): no test;
+//   kCleanZero  a clean grid and a wavefront with slow lanes: q = -0.0, the
+//               one value the bit-pattern bound rejects wrongly, is accepted
+//               by an explicit compare.
+enum CellMode { kWindow = 0, kClean = 1, kCleanZero = 2 };
+
 // Cell of one point under the lane's pose, fast form: valid (in the map),
-// near (some axis needs the exact division, or the lane is slow) and the
-// index to load (the zero cell when not valid).
-template <bool RP0, bool IDX64>
+// near (kWindow: some axis needs the exact division, or the lane is slow) and
+// the index to load (the zero cell when not valid).
+template <bool RP0, bool IDX64, int MODE>
 __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev& g, const ProbSrc<IDX64>& src,
                                                                   const LanePose& lp, float px, float py, float pz,
                                                                   bool& valid, bool& near) {
   float sx, sy, sz;
   rotate<RP0>(lp.r, px, py, pz, sx, sy, sz);
-  bool inx, iny, inz, nx, ny, nz;
+  bool inx, iny, inz;
   uint32_t ix, iy, iz;
-  axis_fast(sx, lp.ox, g.lim[0], g.inv_res, g.near_w[0], g.near_w2[0], inx, nx, ix);
-  axis_fast(sy, lp.oy, g.lim[1], g.inv_res, g.near_w[1], g.near_w2[1], iny, ny, iy);
-  axis_fast(sz, lp.oz, g.lim[2], g.inv_res, g.near_w[2], g.near_w2[2], inz, nz, iz);
+  if constexpr (MODE == kWindow) {
+    bool nx, ny, nz;
+    axis_fast(sx, lp.ox, g.lim[0], g.inv_res, g.near_w[0], g.near_w2[0], inx, nx, ix);
+    axis_fast(sy, lp.oy, g.lim[1], g.inv_res, g.near_w[1], g.near_w2[1], iny, ny, iy);
+    axis_fast(sz, lp.oz, g.lim[2], g.inv_res, g.near_w[2], g.near_w2[2], inz, nz, iz);
+    near = nx | ny | nz | lp.slow;
+  } else {
+    constexpr bool kZero = MODE == kCleanZero;
+    axis_clean<kZero>(sx, lp.ox, g.lim[0], g.inv_res, inx, ix);
+    axis_clean<kZero>(sy, lp.oy, g.lim[1], g.inv_res, iny, iy);
+    axis_clean<kZero>(sz, lp.oz, g.lim[2], g.inv_res, inz, iz);
+    near = false;
+  }
   valid = inx & iny & inz;
-  near = nx | ny | nz | lp.slow;
   const auto lin = linear<IDX64>(g, ix, iy, iz);
   return valid ? lin : src.zero;
 }
@@ -327,7 +346,7 @@ __device__ __forceinline__ float group_member(float v) {
 // per lane and array, then every step reads its point with one LDS read.
 constexpr int kCloudChunk = 64;
 
-template <int Q, bool RP0, bool IDX64>
+template <int Q, bool RP0, bool IDX64, int MODE>
 __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
                                            int& cnt, float4* cl) {
   using Idx = typename ProbSrc<IDX64>::Idx;
@@ -361,6 +380,7 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     wave_lds_sync();  // the previous chunk's reads precede these writes
     cl[lane] = make_float4(f.cx[c0 + lane], f.cy[c0 + lane], f.cz[c0 + lane], 0.f);
     wave_lds_sync();
+#pragma unroll 1
     for (int s0 = 0; s0 < S; s0 += U) {
       Idx idx[U];
       bool nr[U];
@@ -370,13 +390,13 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         bool val;
-        idx[u] = cell_fast<RP0, IDX64>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
+        idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
         cnt += val ? 1 : 0;
       }
       bool any = false;
 #pragma unroll
       for (int u = 0; u < U; ++u) any |= nr[u];
-      if (__builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
+      if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
         // rare: redo the flagged lanes with the division, re-reading their
         // points (nothing of the fast form is kept for this path)
         const volatile float* vcl = reinterpret_cast<const volatile float*>(cl);
@@ -405,8 +425,8 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     // a lane past the cloud's end takes a NaN point, which every test rejects
     const float px = pin ? f.cx[js] : __builtin_nanf(""), py = f.cy[js], pz = f.cz[js];
     bool val, nr;
-    Idx idx = cell_fast<RP0, IDX64>(g, src, lp, px, py, pz, val, nr);
-    if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0, 0)) {
+    Idx idx = cell_fast<RP0, IDX64, MODE>(g, src, lp, px, py, pz, val, nr);
+    if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0, 0)) {
       bool ve;
       const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, px, py, pz, ve);
       idx = nr ? ie : idx;
@@ -417,7 +437,8 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
   }
 }
 
-template <int Q, bool IDX64, bool KFILTER>
+// CLEAN: the launch's grids are all clean (GridDev::clean).
+template <int Q, bool IDX64, bool KFILTER, bool CLEAN>
 __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out,
                              float4* cl) {
   const int lane = lane_id();
@@ -446,10 +467,25 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   if (KFILTER && act && po == 0) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
   float wsum = 0.f;
   int cnt = 0;
-  if (f.rp0)
-    lanes_loop<Q, true, IDX64>(g, f, lp, wsum, cnt, cl);
-  else
-    lanes_loop<Q, false, IDX64>(g, f, lp, wsum, cnt, cl);
+  if constexpr (CLEAN) {
+    const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;  // uniform
+    if (f.rp0) {
+      if (zero)
+        lanes_loop<Q, true, IDX64, kCleanZero>(g, f, lp, wsum, cnt, cl);
+      else
+        lanes_loop<Q, true, IDX64, kClean>(g, f, lp, wsum, cnt, cl);
+    } else {
+      if (zero)
+        lanes_loop<Q, false, IDX64, kCleanZero>(g, f, lp, wsum, cnt, cl);
+      else
+        lanes_loop<Q, false, IDX64, kClean>(g, f, lp, wsum, cnt, cl);
+    }
+  } else {
+    if (f.rp0)
+      lanes_loop<Q, true, IDX64, kWindow>(g, f, lp, wsum, cnt, cl);
+    else
+      lanes_loop<Q, false, IDX64, kWindow>(g, f, lp, wsum, cnt, cl);
+  }
   if (Q > 1) {
     int c = cnt;
 #pragma unroll
@@ -467,40 +503,41 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   }
 }
 
-template <int Q, bool IDX64>
+template <int Q, bool IDX64, bool CLEAN>
 __global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
   constexpr uint32_t P = 64 / Q;
   __shared__ float4 cloud[kWaves * kCloudChunk];
   const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, true>(g, f, p0, min(P, f.n - p0), nullptr, cloud + (threadIdx.x >> 6) * kCloudChunk);
+  weight_lanes<Q, IDX64, true, CLEAN>(g, f, p0, min(P, f.n - p0), nullptr,
+                                      cloud + (threadIdx.x >> 6) * kCloudChunk);
 }
 
-template <int Q, bool IDX64>
+template <int Q, bool IDX64, bool CLEAN>
 __global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
   __shared__ float4 cloud[kWaves * kCloudChunk];
   const uint32_t p0 = wave * P;
   if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, false>(g, f, p0, min(P, f.n - p0), out, cloud + (threadIdx.x >> 6) * kCloudChunk);
+  weight_lanes<Q, IDX64, false, CLEAN>(g, f, p0, min(P, f.n - p0), out, cloud + (threadIdx.x >> 6) * kCloudChunk);
 }
 
 // One work item; a filter that names its own grid (a batch over several
 // maps) is weighted on that grid, else on the launch's.
-template <int Q, bool IDX64>
+template <int Q, bool IDX64, bool CLEAN>
 __device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl) {
   const uint32_t count = min(it.count, (uint32_t)(64 / Q));
   if (f.gdev) {
     const GridDev gf = *f.gdev;
-    weight_lanes<Q, IDX64, true>(gf, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN>(gf, f, it.p0, count, nullptr, cl);
   } else {
-    weight_lanes<Q, IDX64, true>(g, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN>(g, f, it.p0, count, nullptr, cl);
   }
 }
 
-template <int Q, bool IDX64>
+template <int Q, bool IDX64, bool CLEAN>
 __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs, const WorkItem* items,
                                                                uint32_t nitems, uint32_t* counter, int persistent) {
   __shared__ float4 cloud[kWaves * kCloudChunk];
@@ -510,7 +547,7 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
   if (!persistent) {
     if (k >= nitems) return;
     const WorkItem it = items[k];
-    lanes_item<Q, IDX64>(g, fs[it.filter], it, cl);
+    lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cl);
     return;
   }
   for (;;) {
@@ -518,7 +555,7 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
     if (lane == 0) t = atomicAdd(counter, 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
     if (t >= nitems) break;
-    lanes_item<Q, IDX64>(g, fs[items[t].filter], items[t], cl);
+    lanes_item<Q, IDX64, CLEAN>(g, fs[items[t].filter], items[t], cl);
   }
 }
 
@@ -635,15 +672,18 @@ void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* item
     weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
 }
 
+// Launches pick the kernel of the grid's index width and, for a clean grid,
+// the one without the window test.
 template <int Q>
 void lanes_single_q(const GridDev& g, const FilterDev& f, hipStream_t s) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
   const uint32_t blocks = (waves + kWaves - 1) / kWaves;
-  if (g.idx64)
-    lanes_single_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, f);
-  else
-    lanes_single_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, f);
+  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
+  if (k == 0) lanes_single_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, f);
+  else if (k == 1) lanes_single_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, f);
+  else if (k == 2) lanes_single_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, f);
+  else lanes_single_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, f);
 }
 
 template <int Q>
@@ -651,27 +691,33 @@ void lanes_poses_q(const GridDev& g, const FilterDev& f, float* out, hipStream_t
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
   const uint32_t blocks = (waves + kWaves - 1) / kWaves;
-  if (g.idx64)
-    lanes_poses_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, f, out);
-  else
-    lanes_poses_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, f, out);
+  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
+  if (k == 0) lanes_poses_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, f, out);
+  else if (k == 1) lanes_poses_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, f, out);
+  else if (k == 2) lanes_poses_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, f, out);
+  else lanes_poses_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, f, out);
 }
 
+// g.clean must hold for every grid the filters name (the caller clears it
+// otherwise).
 template <int Q>
 void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items, uint32_t nitems, uint32_t* counter,
                    int persistent, hipStream_t s) {
+  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
   uint32_t blocks;
   if (persistent) {
-    static int cus = 0, per_cu[2] = {0, 0};
+    static int cus = 0, per_cu[4] = {0, 0, 0, 0};
     if (!cus) {
       int dev = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false>, kThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, true>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false, false>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, false, true>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], lanes_fleet_kernel<Q, true, false>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[3], lanes_fleet_kernel<Q, true, true>, kThreads, 0);
       if (cus < 1) cus = 256;
     }
-    blocks = (uint32_t)(cus * std::max(1, per_cu[g.idx64 ? 1 : 0]));
+    blocks = (uint32_t)(cus * std::max(1, per_cu[k]));
     const uint32_t need = (nitems + kWaves - 1) / kWaves;
     if (blocks > need) blocks = need;
     if (blocks == 0) blocks = 1;
@@ -680,10 +726,10 @@ void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items,
     blocks = (nitems + kWaves - 1) / kWaves;
   }
   if (blocks == 0) return;
-  if (g.idx64)
-    lanes_fleet_kernel<Q, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else
-    lanes_fleet_kernel<Q, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  if (k == 0) lanes_fleet_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else if (k == 1) lanes_fleet_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else if (k == 2) lanes_fleet_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else lanes_fleet_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
 }
 
 }  // namespace

@@ -211,6 +211,19 @@ def test_cloud_weights_64bit_index_path(gpu, room, shape, monkeypatch):
     assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
+def test_cloud_weights_window_test_on_clean_map(gpu, room, shape, monkeypatch):
+    """The room's 0.1 m grid is clean (the reciprocal product floors like the
+    division for every float), so it normally runs without the window test;
+    the knob forces the window kernels on it."""
+    monkeypatch.setenv("A3D_FORCE_NEAR_TEST", "1")
+    bm = room["map"]
+    g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
+    s, p = room["step"], room["particles"]
+    w = g.cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
 @pytest.fixture(scope="module")
 def long_axis(gpu):
     """A map 15.9 million cells long in x (one cell in y and z) whose resolution
