@@ -346,13 +346,69 @@ __device__ __forceinline__ float group_member(float v) {
 // per lane and array, then every step reads its point with one LDS read.
 constexpr int kCloudChunk = 64;
 
-template <int Q, bool RP0, bool IDX64, int MODE>
-__device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
-                                           int& cnt, float4* cl) {
+// The steps of one staged chunk of 64 points (S = 64/Q steps in groups of U).
+template <int Q, bool RP0, bool IDX64, int MODE, typename Acc>
+__device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX64>& src, const LanePose& lp,
+                                            const float4* cl, int po, int& cnt, Acc&& accumulate) {
   using Idx = typename ProbSrc<IDX64>::Idx;
   constexpr int U = (Q == 1) ? 8 : 4;  // steps in flight
   constexpr int S = kCloudChunk / Q;   // steps per chunk
   static_assert(S % U == 0, "a chunk holds whole groups of steps");
+#pragma unroll 1
+  for (int s0 = 0; s0 < S; s0 += U) {
+    Idx idx[U];
+    bool nr[U];
+    float4 pt[U];  // all LDS reads of the group first
+#pragma unroll
+    for (int u = 0; u < U; ++u) pt[u] = cl[(s0 + u) * Q + po];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bool val;
+      idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
+      cnt += val ? 1 : 0;
+    }
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) any |= nr[u];
+    if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
+      // rare: redo the flagged lanes with the division, re-reading their
+      // points (nothing of the fast form is kept for this path)
+      const volatile float* vcl = reinterpret_cast<const volatile float*>(cl);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (nr[u]) {
+          const int k = 4 * ((s0 + u) * Q + po);
+          bool ve;
+          const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, vcl[k], vcl[k + 1], vcl[k + 2], ve);
+          cnt += (ve ? 1 : 0) - (idx[u] != src.zero ? 1 : 0);
+          idx[u] = ie;
+        }
+      }
+    }
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src(idx[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) accumulate(v[u]);
+  }
+}
+
+__device__ inline float4 cloud_point(const FilterDev& f, uint32_t j) {
+  return make_float4(f.cx[j], f.cy[j], f.cz[j], 0.f);
+}
+
+// One lane's particle over the whole cloud.  BLOCK: the workgroup's
+// wavefronts (all on the same filter) walk the cloud together: 64-point
+// chunks are staged once per workgroup in LDS (double buffered, one barrier
+// per chunk), so the wavefronts' nearby poses gather the same points at the
+// same time and share cache lines; every wavefront of the workgroup must call
+// this (run = false skips the arithmetic of a wavefront with no active
+// particle).  Otherwise each wavefront stages its own chunks (work-queue
+// kernels, whose wavefronts hold different filters).
+template <int Q, bool RP0, bool IDX64, int MODE, bool BLOCK>
+__device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
+                                           int& cnt, float4* cl, bool run) {
+  using Idx = typename ProbSrc<IDX64>::Idx;
   const ProbSrc<IDX64> src(g);
   const int lane = lane_id();
   const int po = lane % Q;
@@ -375,50 +431,35 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
       wsum += group_member<4, 3>(v);
     }
   };
-  uint32_t c0 = 0;
-  for (; c0 + kCloudChunk <= m; c0 += kCloudChunk) {
-    wave_lds_sync();  // the previous chunk's reads precede these writes
-    cl[lane] = make_float4(f.cx[c0 + lane], f.cy[c0 + lane], f.cz[c0 + lane], 0.f);
-    wave_lds_sync();
-#pragma unroll 1
-    for (int s0 = 0; s0 < S; s0 += U) {
-      Idx idx[U];
-      bool nr[U];
-      float4 pt[U];  // all LDS reads of the group first
-#pragma unroll
-      for (int u = 0; u < U; ++u) pt[u] = cl[(s0 + u) * Q + po];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        bool val;
-        idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
-        cnt += val ? 1 : 0;
-      }
-      bool any = false;
-#pragma unroll
-      for (int u = 0; u < U; ++u) any |= nr[u];
-      if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(any) != 0, 0)) {
-        // rare: redo the flagged lanes with the division, re-reading their
-        // points (nothing of the fast form is kept for this path)
-        const volatile float* vcl = reinterpret_cast<const volatile float*>(cl);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (nr[u]) {
-            const int k = 4 * ((s0 + u) * Q + po);
-            bool ve;
-            const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, vcl[k], vcl[k + 1], vcl[k + 2], ve);
-            cnt += (ve ? 1 : 0) - (idx[u] != src.zero ? 1 : 0);
-            idx[u] = ie;
-          }
-        }
-      }
-      float v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = src(idx[u]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) accumulate(v[u]);
+  const uint32_t mfull = m - m % kCloudChunk;
+  if constexpr (BLOCK) {
+    const uint32_t tid = threadIdx.x;
+    if (mfull > 0) {
+      if (tid < (uint32_t)kCloudChunk) cl[tid] = cloud_point(f, tid);
+      __syncthreads();
+    }
+    for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
+      const float4* cur = cl + ((c0 / kCloudChunk) & 1u) * kCloudChunk;
+      float4* nxt = cl + (((c0 / kCloudChunk) & 1u) ^ 1u) * kCloudChunk;
+      const bool more = c0 + kCloudChunk < mfull;
+      float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (more && tid < (uint32_t)kCloudChunk) next = cloud_point(f, c0 + kCloudChunk + tid);
+      if (run) lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cur, po, cnt, accumulate);
+      // every wavefront is done with the buffer written next (it read it in
+      // the previous chunk, before the last barrier)
+      if (more && tid < (uint32_t)kCloudChunk) nxt[tid] = next;
+      __syncthreads();
+    }
+  } else {
+    for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
+      wave_lds_sync();  // the previous chunk's reads precede these writes
+      cl[lane] = cloud_point(f, c0 + lane);
+      wave_lds_sync();
+      lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cl, po, cnt, accumulate);
     }
   }
-  for (; c0 < m; c0 += Q) {  // the last m mod 64 points
+  if (!run) return;
+  for (uint32_t c0 = mfull; c0 < m; c0 += Q) {  // the last m mod 64 points
     const uint32_t j = c0 + po;
     const bool pin = j < m;
     const uint32_t js = pin ? j : 0u;
@@ -437,14 +478,16 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
   }
 }
 
-// CLEAN: the launch's grids are all clean (GridDev::clean).
-template <int Q, bool IDX64, bool KFILTER, bool CLEAN>
+// CLEAN: the launch's grids are all clean (GridDev::clean).  BLOCK: the
+// workgroup walks the cloud together (every wavefront calls this, count may
+// be 0); cl then holds two chunks per workgroup, else one per wavefront.
+template <int Q, bool IDX64, bool KFILTER, bool CLEAN, bool BLOCK>
 __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out,
                              float4* cl) {
   const int lane = lane_id();
   const int ps = lane / Q, po = lane % Q;
   const bool has = (uint32_t)ps < count;
-  const uint32_t pk = p0 + (has ? (uint32_t)ps : 0u);
+  const uint32_t pk = has ? p0 + (uint32_t)ps : 0u;  // f.n >= 1 whenever this runs
   const uint32_t pi = f.order ? f.order[pk] : pk;
   float tx = 0.f, ty = 0.f, tz = 0.f, ta = 0.f;
   if (has) {
@@ -454,7 +497,8 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
     ta = f.a[pi];
   }
   const bool act = KFILTER ? (has && into_map(g, tx, ty, tz)) : has;
-  if (__ballot(act) == 0) return;
+  const bool run = __ballot(act) != 0;
+  if (!BLOCK && !run) return;
   LanePose lp;
   rotation9(f.roll, f.pitch, ta, lp.r);
   const double nan = __builtin_nan("");
@@ -467,24 +511,26 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   if (KFILTER && act && po == 0) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
   float wsum = 0.f;
   int cnt = 0;
+  // the branches are uniform per wavefront; in BLOCK form wavefronts of one
+  // workgroup may take different ones, each passing the same barriers
   if constexpr (CLEAN) {
-    const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;  // uniform
+    const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;
     if (f.rp0) {
       if (zero)
-        lanes_loop<Q, true, IDX64, kCleanZero>(g, f, lp, wsum, cnt, cl);
+        lanes_loop<Q, true, IDX64, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run);
       else
-        lanes_loop<Q, true, IDX64, kClean>(g, f, lp, wsum, cnt, cl);
+        lanes_loop<Q, true, IDX64, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run);
     } else {
       if (zero)
-        lanes_loop<Q, false, IDX64, kCleanZero>(g, f, lp, wsum, cnt, cl);
+        lanes_loop<Q, false, IDX64, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run);
       else
-        lanes_loop<Q, false, IDX64, kClean>(g, f, lp, wsum, cnt, cl);
+        lanes_loop<Q, false, IDX64, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run);
     }
   } else {
     if (f.rp0)
-      lanes_loop<Q, true, IDX64, kWindow>(g, f, lp, wsum, cnt, cl);
+      lanes_loop<Q, true, IDX64, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run);
     else
-      lanes_loop<Q, false, IDX64, kWindow>(g, f, lp, wsum, cnt, cl);
+      lanes_loop<Q, false, IDX64, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run);
   }
   if (Q > 1) {
     int c = cnt;
@@ -503,25 +549,26 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   }
 }
 
+// One filter: the workgroup's wavefronts hold consecutive particles (of the
+// visiting order) and walk the cloud together.
 template <int Q, bool IDX64, bool CLEAN>
 __global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
   constexpr uint32_t P = 64 / Q;
-  __shared__ float4 cloud[kWaves * kCloudChunk];
+  __shared__ float4 cloud[2 * kCloudChunk];
   const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
-  if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, true, CLEAN>(g, f, p0, min(P, f.n - p0), nullptr,
-                                      cloud + (threadIdx.x >> 6) * kCloudChunk);
+  const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
+  weight_lanes<Q, IDX64, true, CLEAN, true>(g, f, p0, count, nullptr, cloud);
 }
 
 template <int Q, bool IDX64, bool CLEAN>
 __global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
   constexpr uint32_t P = 64 / Q;
+  __shared__ float4 cloud[2 * kCloudChunk];
   const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  __shared__ float4 cloud[kWaves * kCloudChunk];
   const uint32_t p0 = wave * P;
-  if (p0 >= f.n) return;
-  weight_lanes<Q, IDX64, false, CLEAN>(g, f, p0, min(P, f.n - p0), out, cloud + (threadIdx.x >> 6) * kCloudChunk);
+  const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
+  weight_lanes<Q, IDX64, false, CLEAN, true>(g, f, p0, count, out, cloud);
 }
 
 // One work item; a filter that names its own grid (a batch over several
@@ -531,9 +578,9 @@ __device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f,
   const uint32_t count = min(it.count, (uint32_t)(64 / Q));
   if (f.gdev) {
     const GridDev gf = *f.gdev;
-    weight_lanes<Q, IDX64, true, CLEAN>(gf, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN, false>(gf, f, it.p0, count, nullptr, cl);
   } else {
-    weight_lanes<Q, IDX64, true, CLEAN>(g, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN, false>(g, f, it.p0, count, nullptr, cl);
   }
 }
 
