@@ -69,6 +69,9 @@ __device__ inline int cvt_i32_sat(double k) {
   return r;
 }
 
+// The axis forms below take the rotated coordinate already widened to double
+// (sd = (double)s, exact).
+//
 // The same axis without branches, for a q that is not -0.0 (the caller
 // routes the lanes where q could be -0.0 to axis_exact):
 //   in   = bits(q) < lim, i.e. 0 <= q < ext && floor(q/res) < size, exactly;
@@ -77,9 +80,9 @@ __device__ inline int cvt_i32_sat(double k) {
 // Away from the window, |q*inv_res - q/res| <= size * 2^-51 < w / 8 keeps
 // both on the same side of every integer, so ix is the reference's index;
 // the caller recomputes the near lanes with axis_exact.
-__device__ inline void axis_fast(float s, double off, uint32_t lim, double inv_res, double w, double w2, bool& in,
+__device__ inline void axis_fast(double sd, double off, uint32_t lim, double inv_res, double w, double w2, bool& in,
                                  bool& near, uint32_t& ix) {
-  const float q = (float)((double)s + off);
+  const float q = (float)(sd + off);
   const double d = (double)q * inv_res;
   ix = (uint32_t)cvt_i32_sat(d);
   near = __builtin_amdgcn_fract(d + w) < w2;  // w2 = 2w
@@ -91,16 +94,16 @@ __device__ inline void axis_fast(float s, double off, uint32_t lim, double inv_r
 // accepts q = -0.0 (for lanes whose offset is within 2^-96 of zero), which
 // lies in cell 0 whenever the axis has a cell (lim > 0).
 template <bool ZERO>
-__device__ inline void axis_clean(float s, double off, uint32_t lim, double inv_res, bool& in, uint32_t& ix) {
-  const float q = (float)((double)s + off);
+__device__ inline void axis_clean(double sd, double off, uint32_t lim, double inv_res, bool& in, uint32_t& ix) {
+  const float q = (float)(sd + off);
   ix = (uint32_t)cvt_i32_sat((double)q * inv_res);
   in = __float_as_uint(q) < lim;
   if (ZERO) in = in | ((q == 0.0f) & (lim != 0u));
 }
 
-__device__ inline void axis_exact(float s, double off, float ext_hi, double res, uint32_t size, bool& ok,
+__device__ inline void axis_exact(double sd, double off, float ext_hi, double res, uint32_t size, bool& ok,
                                   uint32_t& ix) {
-  const float q = (float)((double)s + off);
+  const float q = (float)(sd + off);
   const double k = floor((double)q / res);
   ok = (q >= 0.0f) & (q < ext_hi) & (k < (double)size);
   ix = ok ? (uint32_t)k : 0u;

@@ -46,6 +46,10 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
 constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16-byte rows)
 constexpr int kWaves = 4;      // wavefronts per workgroup
 constexpr int kThreads = 64 * kWaves;
+// Workgroups of the single-filter particle-lane kernels, whose wavefronts
+// walk the cloud together.
+constexpr int kBlockWaves = 4;
+constexpr int kBlockThreads = 64 * kBlockWaves;
 
 // Cell of one point under one pose: returns whether it is in the map and
 // its index (0 when not, so that the gather is always in bounds).
@@ -243,22 +247,34 @@ struct LanePose {
   bool slow;
 };
 
-// With roll = pitch = 0 the terms of the reference's sums that are products
-// with r02 = r12 = r20 = r21 = +-0 are dropped, which is exact for a finite
-// point; a non-finite coordinate makes sz (pz itself) or one of sx, sy (px,
-// py times cos(yaw), which is never 0 for a float yaw) non-finite, so the
-// point fails an axis test, as in the reference.
+// A cloud point as staged for the particle lanes: x and y as floats and z as
+// a float in .z, or, with roll = pitch = 0 (where z enters the arithmetic
+// only as (double)pz), as that double in (.z, .w).
 template <bool RP0>
-__device__ __forceinline__ void rotate(const float (&r)[9], float px, float py, float pz, float& sx, float& sy,
-                                       float& sz) {
+__device__ __forceinline__ float4 stage_point(float px, float py, float pz) {
   if (RP0) {
-    sx = px * r[0] + py * r[1];
-    sy = px * r[3] + py * r[4];
-    sz = pz;
+    const double d = (double)pz;
+    return make_float4(px, py, __int_as_float(__double2loint(d)), __int_as_float(__double2hiint(d)));
+  }
+  return make_float4(px, py, pz, 0.f);
+}
+
+// The rotated point of Grid3d.cpp:163-172, widened to double.  With roll =
+// pitch = 0 the terms of the reference's sums that are products with r02 =
+// r12 = r20 = r21 = +-0 are dropped, which is exact for a finite point; a
+// non-finite coordinate makes sz (pz itself) or one of sx, sy (px, py times
+// cos(yaw), which is never 0 for a float yaw) non-finite, so the point fails
+// an axis test, as in the reference.
+template <bool RP0>
+__device__ __forceinline__ void rotate(const float (&r)[9], const float4& p, double& dx, double& dy, double& dz) {
+  if (RP0) {
+    dx = (double)(p.x * r[0] + p.y * r[1]);
+    dy = (double)(p.x * r[3] + p.y * r[4]);
+    dz = __hiloint2double(__float_as_int(p.w), __float_as_int(p.z));
   } else {
-    sx = px * r[0] + py * r[1] + pz * r[2];
-    sy = px * r[3] + py * r[4] + pz * r[5];
-    sz = px * r[6] + py * r[7] + pz * r[8];
+    dx = (double)(p.x * r[0] + p.y * r[1] + p.z * r[2]);
+    dy = (double)(p.x * r[3] + p.y * r[4] + p.z * r[5]);
+    dz = (double)(p.x * r[6] + p.y * r[7] + p.z * r[8]);
   }
 }
 
@@ -278,6 +294,22 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx linear(const GridDev& g,
   }
 }
 
+// Lane-mask forms of a select and a count: the mask is the one a compare
+// writes (inactive lanes 0), so each costs one vector instruction (the mask
+// is the instruction's one scalar operand).
+__device__ __forceinline__ uint32_t select_by_mask(uint64_t m, uint32_t if_set, uint32_t if_clear) {
+  uint32_t r;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(if_clear), "v"(if_set), "s"(m));
+  return r;
+}
+
+__device__ __forceinline__ int add_mask_bit(int c, uint64_t m) {  // c + (this lane's bit of m)
+  int r;
+  uint64_t carry;
+  asm("v_addc_co_u32_e64 %0, %1, 0, %2, %3" : "=v"(r), "=s"(carry) : "v"(c), "s"(m));
+  return r;
+}
+
 // How the fast cell index is made exact:
 //   kWindow     window test on every axis, flagged lanes (and slow lanes)
 //               redone with the division;
@@ -288,47 +320,53 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx linear(const GridDev& g,
 //               by an explicit compare.
 enum CellMode { kWindow = 0, kClean = 1, kCleanZero = 2 };
 
-// Cell of one point under the lane's pose, fast form: valid (in the map),
-// near (kWindow: some axis needs the exact division, or the lane is slow) and
-// the index to load (the zero cell when not valid).
+// Cell of one staged point under the lane's pose, fast form: vm, the lanes
+// whose point is in the map; near (kWindow: some axis needs the exact
+// division, or the lane is slow); and the index to load (the zero cell
+// outside the map).
 template <bool RP0, bool IDX64, int MODE>
 __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev& g, const ProbSrc<IDX64>& src,
-                                                                  const LanePose& lp, float px, float py, float pz,
-                                                                  bool& valid, bool& near) {
-  float sx, sy, sz;
-  rotate<RP0>(lp.r, px, py, pz, sx, sy, sz);
+                                                                  const LanePose& lp, const float4& p, uint64_t& vm,
+                                                                  bool& near) {
+  double dx, dy, dz;
+  rotate<RP0>(lp.r, p, dx, dy, dz);
   bool inx, iny, inz;
   uint32_t ix, iy, iz;
   if constexpr (MODE == kWindow) {
     bool nx, ny, nz;
-    axis_fast(sx, lp.ox, g.lim[0], g.inv_res, g.near_w[0], g.near_w2[0], inx, nx, ix);
-    axis_fast(sy, lp.oy, g.lim[1], g.inv_res, g.near_w[1], g.near_w2[1], iny, ny, iy);
-    axis_fast(sz, lp.oz, g.lim[2], g.inv_res, g.near_w[2], g.near_w2[2], inz, nz, iz);
+    axis_fast(dx, lp.ox, g.lim[0], g.inv_res, g.near_w[0], g.near_w2[0], inx, nx, ix);
+    axis_fast(dy, lp.oy, g.lim[1], g.inv_res, g.near_w[1], g.near_w2[1], iny, ny, iy);
+    axis_fast(dz, lp.oz, g.lim[2], g.inv_res, g.near_w[2], g.near_w2[2], inz, nz, iz);
     near = nx | ny | nz | lp.slow;
   } else {
     constexpr bool kZero = MODE == kCleanZero;
-    axis_clean<kZero>(sx, lp.ox, g.lim[0], g.inv_res, inx, ix);
-    axis_clean<kZero>(sy, lp.oy, g.lim[1], g.inv_res, iny, iy);
-    axis_clean<kZero>(sz, lp.oz, g.lim[2], g.inv_res, inz, iz);
+    axis_clean<kZero>(dx, lp.ox, g.lim[0], g.inv_res, inx, ix);
+    axis_clean<kZero>(dy, lp.oy, g.lim[1], g.inv_res, iny, iy);
+    axis_clean<kZero>(dz, lp.oz, g.lim[2], g.inv_res, inz, iz);
     near = false;
   }
-  valid = inx & iny & inz;
   const auto lin = linear<IDX64>(g, ix, iy, iz);
-  return valid ? lin : src.zero;
+  if constexpr (IDX64) {
+    const bool valid = inx & iny & inz;
+    vm = __builtin_amdgcn_ballot_w64(valid);
+    return valid ? lin : src.zero;
+  } else {
+    vm = __builtin_amdgcn_ballot_w64(inx) & __builtin_amdgcn_ballot_w64(iny) & __builtin_amdgcn_ballot_w64(inz);
+    return select_by_mask(vm, lin, src.zero);
+  }
 }
 
 // The same with the correctly rounded division on every axis.
 template <bool RP0, bool IDX64>
 __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_exact(const GridDev& g, const ProbSrc<IDX64>& src,
-                                                                   const LanePose& lp, float px, float py, float pz,
-                                                                   bool& valid) {
-  float sx, sy, sz;
-  rotate<RP0>(lp.r, px, py, pz, sx, sy, sz);
+                                                                   const LanePose& lp, const float4& p, bool& valid) {
+  double dx, dy, dz;
+  rotate<RP0>(lp.r, p, dx, dy, dz);
   bool okx, oky, okz;
   uint32_t ix, iy, iz;
-  axis_exact(sx, lp.ox, g.ext_hi[0], g.res, g.size[0], okx, ix);
-  axis_exact(sy, lp.oy, g.ext_hi[1], g.res, g.size[1], oky, iy);
-  axis_exact(sz, lp.oz, g.ext_hi[2], g.res, g.size[2], okz, iz);
+  axis_exact(dx, lp.ox, g.ext_hi[0], g.res, g.size[0], okx, ix);
+  axis_exact(dy, lp.oy, g.ext_hi[1], g.res, g.size[1], oky, iy);
+  axis_exact(dz, lp.oz, g.ext_hi[2], g.res, g.size[2], okz, iz);
   valid = okx & oky & okz;
   const auto lin = linear<IDX64>(g, ix, iy, iz);
   return valid ? lin : src.zero;
@@ -363,9 +401,9 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX6
     for (int u = 0; u < U; ++u) pt[u] = cl[(s0 + u) * Q + po];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      bool val;
-      idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u].x, pt[u].y, pt[u].z, val, nr[u]);
-      cnt += val ? 1 : 0;
+      uint64_t vm;
+      idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u], vm, nr[u]);
+      cnt = add_mask_bit(cnt, vm);
     }
     bool any = false;
 #pragma unroll
@@ -378,8 +416,9 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX6
       for (int u = 0; u < U; ++u) {
         if (nr[u]) {
           const int k = 4 * ((s0 + u) * Q + po);
+          const float4 p = make_float4(vcl[k], vcl[k + 1], vcl[k + 2], vcl[k + 3]);
           bool ve;
-          const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, vcl[k], vcl[k + 1], vcl[k + 2], ve);
+          const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, p, ve);
           cnt += (ve ? 1 : 0) - (idx[u] != src.zero ? 1 : 0);
           idx[u] = ie;
         }
@@ -393,8 +432,9 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX6
   }
 }
 
+template <bool RP0>
 __device__ inline float4 cloud_point(const FilterDev& f, uint32_t j) {
-  return make_float4(f.cx[j], f.cy[j], f.cz[j], 0.f);
+  return stage_point<RP0>(f.cx[j], f.cy[j], f.cz[j]);
 }
 
 // One lane's particle over the whole cloud.  BLOCK: the workgroup's
@@ -435,7 +475,7 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
   if constexpr (BLOCK) {
     const uint32_t tid = threadIdx.x;
     if (mfull > 0) {
-      if (tid < (uint32_t)kCloudChunk) cl[tid] = cloud_point(f, tid);
+      if (tid < (uint32_t)kCloudChunk) cl[tid] = cloud_point<RP0>(f, tid);
       __syncthreads();
     }
     for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
@@ -443,7 +483,7 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
       float4* nxt = cl + (((c0 / kCloudChunk) & 1u) ^ 1u) * kCloudChunk;
       const bool more = c0 + kCloudChunk < mfull;
       float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (more && tid < (uint32_t)kCloudChunk) next = cloud_point(f, c0 + kCloudChunk + tid);
+      if (more && tid < (uint32_t)kCloudChunk) next = cloud_point<RP0>(f, c0 + kCloudChunk + tid);
       if (run) lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cur, po, cnt, accumulate);
       // every wavefront is done with the buffer written next (it read it in
       // the previous chunk, before the last barrier)
@@ -453,7 +493,7 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
   } else {
     for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
       wave_lds_sync();  // the previous chunk's reads precede these writes
-      cl[lane] = cloud_point(f, c0 + lane);
+      cl[lane] = cloud_point<RP0>(f, c0 + lane);
       wave_lds_sync();
       lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cl, po, cnt, accumulate);
     }
@@ -464,16 +504,19 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     const bool pin = j < m;
     const uint32_t js = pin ? j : 0u;
     // a lane past the cloud's end takes a NaN point, which every test rejects
-    const float px = pin ? f.cx[js] : __builtin_nanf(""), py = f.cy[js], pz = f.cz[js];
-    bool val, nr;
-    Idx idx = cell_fast<RP0, IDX64, MODE>(g, src, lp, px, py, pz, val, nr);
+    const float4 p = stage_point<RP0>(pin ? f.cx[js] : __builtin_nanf(""), f.cy[js], f.cz[js]);
+    uint64_t vm;
+    bool nr;
+    Idx idx = cell_fast<RP0, IDX64, MODE>(g, src, lp, p, vm, nr);
+    cnt = add_mask_bit(cnt, vm);
     if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0, 0)) {
-      bool ve;
-      const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, px, py, pz, ve);
-      idx = nr ? ie : idx;
-      val = nr ? ve : val;
+      if (nr) {
+        bool ve;
+        const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, p, ve);
+        cnt += (ve ? 1 : 0) - (idx != src.zero ? 1 : 0);
+        idx = ie;
+      }
     }
-    cnt += val ? 1 : 0;
     accumulate(src(idx));
   }
 }
@@ -552,20 +595,20 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
 // One filter: the workgroup's wavefronts hold consecutive particles (of the
 // visiting order) and walk the cloud together.
 template <int Q, bool IDX64, bool CLEAN>
-__global__ __launch_bounds__(kThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
+__global__ __launch_bounds__(kBlockThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
   constexpr uint32_t P = 64 / Q;
   __shared__ float4 cloud[2 * kCloudChunk];
-  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+  const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kBlockWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
   weight_lanes<Q, IDX64, true, CLEAN, true>(g, f, p0, count, nullptr, cloud);
 }
 
 template <int Q, bool IDX64, bool CLEAN>
-__global__ __launch_bounds__(kThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
+__global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
   constexpr uint32_t P = 64 / Q;
   __shared__ float4 cloud[2 * kCloudChunk];
-  const uint32_t wave = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const uint32_t wave = blockIdx.x * kBlockWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
   weight_lanes<Q, IDX64, false, CLEAN, true>(g, f, p0, count, out, cloud);
@@ -725,24 +768,24 @@ template <int Q>
 void lanes_single_q(const GridDev& g, const FilterDev& f, hipStream_t s) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
-  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  const uint32_t blocks = (waves + kBlockWaves - 1) / kBlockWaves;
   const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
-  if (k == 0) lanes_single_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, f);
-  else if (k == 1) lanes_single_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, f);
-  else if (k == 2) lanes_single_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, f);
-  else lanes_single_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, f);
+  if (k == 0) lanes_single_kernel<Q, false, false><<<blocks, kBlockThreads, 0, s>>>(g, f);
+  else if (k == 1) lanes_single_kernel<Q, false, true><<<blocks, kBlockThreads, 0, s>>>(g, f);
+  else if (k == 2) lanes_single_kernel<Q, true, false><<<blocks, kBlockThreads, 0, s>>>(g, f);
+  else lanes_single_kernel<Q, true, true><<<blocks, kBlockThreads, 0, s>>>(g, f);
 }
 
 template <int Q>
 void lanes_poses_q(const GridDev& g, const FilterDev& f, float* out, hipStream_t s) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
-  const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+  const uint32_t blocks = (waves + kBlockWaves - 1) / kBlockWaves;
   const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
-  if (k == 0) lanes_poses_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, f, out);
-  else if (k == 1) lanes_poses_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, f, out);
-  else if (k == 2) lanes_poses_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, f, out);
-  else lanes_poses_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, f, out);
+  if (k == 0) lanes_poses_kernel<Q, false, false><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
+  else if (k == 1) lanes_poses_kernel<Q, false, true><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
+  else if (k == 2) lanes_poses_kernel<Q, true, false><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
+  else lanes_poses_kernel<Q, true, true><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
 }
 
 // g.clean must hold for every grid the filters name (the caller clears it
