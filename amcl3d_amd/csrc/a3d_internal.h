@@ -151,7 +151,8 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g);
 // ------------------------------------------------------------ launchers
 // Cloud weight (Grid3d::computeCloudWeight for every in-map particle) and the
 // range weight; writes wp, wr of in-map particles (out-of-map ones keep their
-// previous wp, wr as in ParticleFilter.cpp:137-141).
+// previous wp, wr as in ParticleFilter.cpp:137-141).  A persistent launch
+// over a work list uses d_counter[0..7] (one queue counter per XCD).
 void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
                    uint32_t* d_counter, int ppw, int persistent, hipStream_t s);
 // Poses given explicitly (a3d_grid_cloud_weight): out[i].

@@ -43,6 +43,13 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
+// Work-queue kernels split the (filter-major) item list into one contiguous
+// range per XCD, each with its own counter: workgroup b starts on range b mod
+// 8, the one of its own XCD, so an L2 serves the particles of few filters at a
+// time, and moves on to the other ranges (in turn) when its own is empty.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_range(uint32_t n, uint32_t x) { return (uint32_t)((uint64_t)n * x / kXcds); }
+
 constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16-byte rows)
 constexpr int kWaves = 4;      // wavefronts per workgroup
 constexpr int kThreads = 64 * kWaves;
@@ -640,12 +647,15 @@ __global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const 
     lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cl);
     return;
   }
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(counter, 1u);
-    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-    if (t >= nitems) break;
-    lanes_item<Q, IDX64, CLEAN>(g, fs[items[t].filter], items[t], cl);
+  for (uint32_t r = 0; r < kXcds; ++r) {
+    const uint32_t x = (blockIdx.x + r) % kXcds, beg = xcd_range(nitems, x), len = xcd_range(nitems, x + 1) - beg;
+    for (;;) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(counter + x, 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      if (t >= len) break;
+      lanes_item<Q, IDX64, CLEAN>(g, fs[items[beg + t].filter], items[beg + t], cl);
+    }
   }
 }
 
@@ -700,12 +710,15 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
     block_item<PB, IDX64>(g, fs[items[k].filter], items[k], tile);
     return;
   }
-  for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(counter, 1u);
-    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-    if (t >= nitems) break;
-    block_item<PB, IDX64>(g, fs[items[t].filter], items[t], tile);
+  for (uint32_t r = 0; r < kXcds; ++r) {
+    const uint32_t x = (blockIdx.x + r) % kXcds, beg = xcd_range(nitems, x), len = xcd_range(nitems, x + 1) - beg;
+    for (;;) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(counter + x, 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      if (t >= len) break;
+      block_item<PB, IDX64>(g, fs[items[beg + t].filter], items[beg + t], tile);
+    }
   }
 }
 
@@ -751,7 +764,7 @@ void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* item
     const uint32_t need = (nitems + kWaves - 1) / kWaves;
     if (blocks > need) blocks = need;
     if (blocks == 0) blocks = 1;
-    zero_u32_kernel<<<1, 64, 0, s>>>(counter, 1);
+    zero_u32_kernel<<<1, 64, 0, s>>>(counter, kXcds);
   } else {
     blocks = (nitems + kWaves - 1) / kWaves;
   }
@@ -811,7 +824,7 @@ void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items,
     const uint32_t need = (nitems + kWaves - 1) / kWaves;
     if (blocks > need) blocks = need;
     if (blocks == 0) blocks = 1;
-    zero_u32_kernel<<<1, 64, 0, s>>>(counter, 1);
+    zero_u32_kernel<<<1, 64, 0, s>>>(counter, kXcds);
   } else {
     blocks = (nitems + kWaves - 1) / kWaves;
   }
