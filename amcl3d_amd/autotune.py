@@ -6,8 +6,8 @@ A shape is (mapping, particles per wavefront, visiting order):
   * mapping 1 (MAP_POINT_LANES): the wavefront's 64 lanes are 64 consecutive
     points, transformed for ppw particles through an LDS tile;
   * order 0 visits particles in index order, order 1 in a spatial order
-    computed on the device before each single-filter update (fleets and
-    served batches visit in index order).
+    computed on the device before each weight launch (per filter, also in
+    fleets and served batches).
 Every shape gives bit-identical results, so the choice is purely a timing
 one: run the caller's workload once per candidate (after one untimed warm-up)
 and keep the fastest.
@@ -17,7 +17,7 @@ import time
 from . import _lib
 
 CANDIDATES = tuple((m, p, o) for o in (1, 0) for m, p in ((0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)))
-FLEET_CANDIDATES = ((0, 64, 0), (0, 32, 0), (0, 16, 0), (1, 16, 0), (1, 32, 0), (1, 64, 0))
+FLEET_CANDIDATES = CANDIDATES
 
 
 def set_shape(shape):

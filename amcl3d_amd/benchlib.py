@@ -1,9 +1,10 @@
 """Workloads of the batched and served forms, for bench.py.
 
-batched: 256 independent filters on the room map (200 x 200 x 60 cells at
-0.1 m); per filter N log-uniform in 500..50,000 particles, M uniform in
-1,000..30,000 points, 1..8 beacons, all seeded.  One persistent launch per
-update versus the plain single-filter kernel launched once per filter.
+batched: 256 independent filters sharing the single-large-filter map (the
+100 x 100 x 30 m building at 0.05 m, 2000 x 2000 x 600 cells); per filter N
+log-uniform in 500..50,000 particles, M uniform in 1,000..30,000 points, 1..8
+beacons, all seeded.  One persistent launch per update versus the plain
+single-filter kernel launched once per filter.
 
 served: sessions on resident maps, each sending Poisson update requests at
 10 Hz; per session N log-uniform in 1,000..50,000, M uniform in
@@ -74,7 +75,7 @@ def fleet_case(W, bm, nfilters, seed):
     return counts, parts, clouds, ranges
 
 
-def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3):
+def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3, map_name="the building map"):
     counts, parts, clouds, ranges = fleet_case(W, bm, nfilters, seed)
     fl = fleet_mod.Fleet(counts)
     fl.set_particles(np.concatenate(parts))
@@ -99,8 +100,7 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3):
     t_loop = min(_timed(run_loop) for _ in range(reps))
     ev = fl.evals
     fl.close()
-    return {"workload": "%d filters on the 200x200x60 room map, N log-uniform 500..50k, M 1k..30k, 1..8 beacons"
-                        % nfilters,
+    return {"workload": "%d filters on %s, N log-uniform 500..50k, M 1k..30k, 1..8 beacons" % (nfilters, map_name),
             "evals_per_update": ev, "shape": autotune.shape_name(shape),
             "autotune_s": {autotune.shape_name(k): v for k, v in timings.items()},
             "batched_evals_per_s": ev / t_batched, "plain_loop_evals_per_s": ev / t_loop,
@@ -226,8 +226,12 @@ def serving_maps(amcl3d_amd, W, building=None, log=print):
 
 
 def extras(amcl3d_amd, W, plain_value=None, building=None, log=print):
-    bm, grid = room_grid(amcl3d_amd, W)
-    out = {"batched": batched(amcl3d_amd, W, bm, grid)}
+    if building is not None:
+        out = {"batched": batched(amcl3d_amd, W, building[0], building[1],
+                                  map_name="the 100x100x30 m building at 0.05 m (2000x2000x600 cells)")}
+    else:
+        bm, grid = room_grid(amcl3d_amd, W)
+        out = {"batched": batched(amcl3d_amd, W, bm, grid, map_name="the 200x200x60 room map")}
     log("batched:", out["batched"]["batched_evals_per_s"], "evals/s")
     maps = serving_maps(amcl3d_amd, W, building, log=log)
     # the serving workload as specified: 1,024 sessions at 10 Hz each

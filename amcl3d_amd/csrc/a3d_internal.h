@@ -136,6 +136,8 @@ struct OrderDev {
   uint32_t* hist;   // kOrderKeys + 1
 };
 inline size_t order_bytes(uint32_t n) { return 64 * 8 * 8 + (2 * (size_t)n + kOrderKeys + 1) * 4; }
+// order_bytes rounded up to keep consecutive views 256-byte aligned
+inline size_t order_stride(uint32_t n) { return (order_bytes(n) + 255) & ~(size_t)255; }
 inline OrderDev order_view(void* base, uint32_t n) {
   OrderDev o;
   o.part = static_cast<double*>(base);
@@ -170,6 +172,9 @@ void launch_gather_means(const Filters& fs, float* out, hipStream_t s);
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 // Visiting order of f's particles for the cloud weight, into o.order.
 void launch_order(const FilterDev& f, const OrderDev& o, hipStream_t s);
+// The same for count filters at once (device arrays fs[i], os[i]; max_n the
+// largest particle count).
+void launch_order_many(const FilterDev* fs, const OrderDev* os, uint32_t count, uint32_t max_n, hipStream_t s);
 
 // Device map build (computeGrid, PointCloudTools.cpp:84-149).
 int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_prob, float* d_dist,

@@ -42,6 +42,12 @@ struct a3d_fleet {
   size_t items_cap = 0;
   uint32_t nitems = 0;
   int items_ppw = 32;  // particles per work item of the staged list
+  bool items_spatial = false;  // the staged filters visit particles in spatial order
+  // particle visiting orders: one view per filter in one allocation
+  uint8_t* ord = nullptr;
+  std::vector<uint64_t> ooff;  // F + 1, bytes
+  OrderDev* d_ord = nullptr;
+  std::vector<OrderDev> h_ord;
   uint32_t* d_counter = nullptr;
   bool staged = false;
   bool prefix_valid = false;  // the last update left resample's running sum c
@@ -126,6 +132,17 @@ int a3d_fleet_create(size_t nfilters, const uint32_t* num_particles, a3d_fleet**
             hipMalloc(&fl->inmap, fl->total) == hipSuccess && hipMalloc(&fl->mean, 16 * nfilters) == hipSuccess &&
             hipMalloc(&fl->d_filters, sizeof(FilterDev) * nfilters) == hipSuccess &&
             hipMalloc(&fl->d_counter, 64) == hipSuccess;
+  fl->ooff.assign(nfilters + 1, 0);
+  for (size_t f = 0; f < nfilters; ++f) fl->ooff[f + 1] = fl->ooff[f] + order_stride(fl->n[f]);
+  if (ok)
+    ok = hipMalloc(&fl->ord, fl->ooff[nfilters]) == hipSuccess &&
+         hipMalloc(&fl->d_ord, sizeof(OrderDev) * nfilters) == hipSuccess;
+  if (ok) {
+    fl->h_ord.resize(nfilters);
+    for (size_t f = 0; f < nfilters; ++f) fl->h_ord[f] = order_view(fl->ord + fl->ooff[f], fl->n[f]);
+    ok = hipMemcpyAsync(fl->d_ord, fl->h_ord.data(), sizeof(OrderDev) * nfilters, hipMemcpyHostToDevice,
+                        fl->stream) == hipSuccess;
+  }
   if (ok)
     ok = hipMemsetAsync(fl->state, 0, 7 * fl->total * 4, fl->stream) == hipSuccess &&
          hipMemsetAsync(fl->mean, 0, 16 * nfilters, fl->stream) == hipSuccess &&
@@ -149,6 +166,8 @@ int a3d_fleet_destroy(a3d_fleet* fl) {
   (void)hipFree(fl->mean);
   (void)hipFree(fl->d_filters);
   (void)hipFree(fl->d_counter);
+  (void)hipFree(fl->ord);
+  (void)hipFree(fl->d_ord);
   (void)hipFree(fl->stage);
   (void)hipFree(fl->cloud);
   (void)hipFree(fl->d_items);
@@ -266,6 +285,9 @@ int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const 
     d.rp0 = (d.roll == 0.0f && d.pitch == 0.0f) ? 1u : 0u;
     fl->h_filters[f] = d;
   }
+  // particles visited in spatial order (computed per update, before the weight launch)
+  fl->items_spatial = current_order() == A3D_ORDER_SPATIAL;
+  for (uint32_t f = 0; f < fl->F; ++f) fl->h_filters[f].order = fl->items_spatial ? fl->h_ord[f].order : nullptr;
   rc = upload_filters(fl);
   if (rc) return rc;
   // Work list: blocks of particles, filters in decreasing cost n*m.
@@ -305,12 +327,15 @@ int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent) {
   if (ev.first) (void)hipEventRecord(ev.first, fl->stream);
   if (persistent >= 0) {
     // the list was cut for the tile current at staging time; launch with that tile
+    if (fl->items_spatial) launch_order_many(fl->d_filters, fl->d_ord, fl->F, fl->max_n, fl->stream);
     launch_weight(g->g, fl->device_filters(), fl->d_items, fl->nitems, fl->d_counter, fl->items_ppw, persistent,
                   fl->stream);
   } else {
-    // the plain single-filter kernel, launched once per filter
-    for (uint32_t f = 0; f < fl->F; ++f)
+    // the plain single-filter path (a3d_pf_update's launches), once per filter
+    for (uint32_t f = 0; f < fl->F; ++f) {
+      if (fl->items_spatial) launch_order(fl->h_filters[f], fl->h_ord[f], fl->stream);
       launch_weight(g->g, single(fl->h_filters[f]), nullptr, 0, nullptr, current_ppw(), 0, fl->stream);
+    }
   }
   if (ev.first) {
     (void)hipEventRecord(ev.second, fl->stream);

@@ -12,12 +12,15 @@
 // code of (x, y, z, yaw), each quantised to 8 levels of half a standard
 // deviation around the particles' mean (so the levels stay populated however
 // concentrated the set is).  Four kernels, no library calls and no memset,
-// so the sequence can be captured into a HIP graph:
+// so the sequence can be captured into a HIP graph; the batched form sorts
+// many filters at once, one filter per grid row (blockIdx.y):
 //   moments  64 blocks: partial sums of x, y, z, a and their squares; clears the histogram
 //   keys     every block derives the levels from the partial sums; key per
 //            particle, histogram per block in LDS, added to the global one
 //   scan     one workgroup: exclusive scan of the 4,096 counts
 //   scatter  order[offset[key]++] = i (order inside a key is arbitrary)
+#include <algorithm>
+
 #include "a3d_math.h"
 
 namespace a3d {
@@ -38,9 +41,34 @@ __device__ inline uint32_t level8(float v, float lo, float scale) {
 constexpr int kMB = 64;   // blocks of the moments kernel
 constexpr int kKT = 256;  // threads per block of the keys and scatter kernels
 
+// The filter of this block: the launch's one, or row base + blockIdx.y of
+// the arrays.
+struct OrderJob {
+  FilterDev f;
+  OrderDev o;
+  const FilterDev* fs;
+  const OrderDev* os;
+  uint32_t base;
+};
+
+template <bool MANY>
+__device__ inline void job_filter(const OrderJob& j, FilterDev& f, OrderDev& o) {
+  if (MANY) {
+    f = j.fs[j.base + blockIdx.y];
+    o = j.os[j.base + blockIdx.y];
+  } else {
+    f = j.f;
+    o = j.o;
+  }
+}
+
 // Partial sums of x, y, z, a and their squares (in double) per block, and the
 // histogram cleared for the keys kernel.
-__global__ __launch_bounds__(kKT) void order_moments_kernel(FilterDev f, OrderDev o) {
+template <bool MANY>
+__global__ __launch_bounds__(kKT) void order_moments_kernel(OrderJob j) {
+  FilterDev f;
+  OrderDev o;
+  job_filter<MANY>(j, f, o);
   __shared__ double red[8][kKT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t gt = blockIdx.x * kKT + tid, gs = kMB * kKT;
@@ -99,7 +127,12 @@ __device__ inline uint32_t order_key(const FilterDev& f, uint32_t i, const float
 
 // Keys, counted per block in LDS and added to the global histogram once per
 // occupied key (few global atomics however concentrated the particles are).
-__global__ __launch_bounds__(kKT) void order_keys_kernel(FilterDev f, OrderDev o) {
+template <bool MANY>
+__global__ __launch_bounds__(kKT) void order_keys_kernel(OrderJob j) {
+  FilterDev f;
+  OrderDev o;
+  job_filter<MANY>(j, f, o);
+  if (blockIdx.x * kKT >= f.n) return;  // whole block past this filter's end (batched launches)
   __shared__ uint32_t h[kOrderKeys];
   __shared__ float box[8];
   for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT) h[k] = 0u;
@@ -116,7 +149,11 @@ __global__ __launch_bounds__(kKT) void order_keys_kernel(FilterDev f, OrderDev o
     if (h[k]) atomicAdd(&o.hist[k], h[k]);
 }
 
-__global__ __launch_bounds__(kBT) void order_scan_kernel(OrderDev o) {
+template <bool MANY>
+__global__ __launch_bounds__(kBT) void order_scan_kernel(OrderJob j) {
+  FilterDev f;
+  OrderDev o;
+  job_filter<MANY>(j, f, o);
   // kOrderKeys / kBT consecutive counts per thread
   __shared__ uint32_t part[kBT];
   constexpr int kPer = kOrderKeys / kBT;
@@ -142,7 +179,12 @@ __global__ __launch_bounds__(kBT) void order_scan_kernel(OrderDev o) {
 
 // order[offset[key]++] = i: a block reserves one range per occupied key and
 // places its particles of that key inside it (in arbitrary order).
-__global__ __launch_bounds__(kKT) void order_scatter_kernel(FilterDev f, OrderDev o) {
+template <bool MANY>
+__global__ __launch_bounds__(kKT) void order_scatter_kernel(OrderJob j) {
+  FilterDev f;
+  OrderDev o;
+  job_filter<MANY>(j, f, o);
+  if (blockIdx.x * kKT >= f.n) return;  // whole block past this filter's end (batched launches)
   __shared__ uint32_t cnt[kOrderKeys];
   for (uint32_t k = threadIdx.x; k < kOrderKeys; k += kKT) cnt[k] = 0u;
   __syncthreads();
@@ -164,10 +206,24 @@ __global__ __launch_bounds__(kKT) void order_scatter_kernel(FilterDev f, OrderDe
 void launch_order(const FilterDev& f, const OrderDev& o, hipStream_t s) {
   if (f.n == 0) return;
   const uint32_t blocks = (f.n + kKT - 1) / kKT;
-  order_moments_kernel<<<kMB, kKT, 0, s>>>(f, o);
-  order_keys_kernel<<<blocks, kKT, 0, s>>>(f, o);
-  order_scan_kernel<<<1, kBT, 0, s>>>(o);
-  order_scatter_kernel<<<blocks, kKT, 0, s>>>(f, o);
+  const OrderJob j{f, o, nullptr, nullptr, 0};
+  order_moments_kernel<false><<<kMB, kKT, 0, s>>>(j);
+  order_keys_kernel<false><<<blocks, kKT, 0, s>>>(j);
+  order_scan_kernel<false><<<1, kBT, 0, s>>>(j);
+  order_scatter_kernel<false><<<blocks, kKT, 0, s>>>(j);
+}
+
+void launch_order_many(const FilterDev* fs, const OrderDev* os, uint32_t count, uint32_t max_n, hipStream_t s) {
+  if (count == 0 || max_n == 0) return;
+  const uint32_t blocks = (max_n + kKT - 1) / kKT;
+  for (uint32_t base = 0; base < count; base += 65535u) {
+    const uint32_t rows = std::min(count - base, 65535u);
+    const OrderJob j{FilterDev{}, OrderDev{}, fs, os, base};
+    order_moments_kernel<true><<<dim3(kMB, rows), kKT, 0, s>>>(j);
+    order_keys_kernel<true><<<dim3(blocks, rows), kKT, 0, s>>>(j);
+    order_scan_kernel<true><<<dim3(1, rows), kBT, 0, s>>>(j);
+    order_scatter_kernel<true><<<dim3(blocks, rows), kKT, 0, s>>>(j);
+  }
 }
 
 }  // namespace a3d

@@ -51,8 +51,11 @@ struct Session {
   float* scratch = nullptr;
   uint8_t* inmap = nullptr;
   float* mean = nullptr;
+  uint8_t* ord = nullptr;  // visiting order buffers (order_view)
   size_t bytes = 0;
   bool busy = false;
+
+  OrderDev ov() const { return order_view(ord, n); }
 
   FilterDev dev() const {
     FilterDev d{};
@@ -112,6 +115,8 @@ struct Slot {
   size_t z4_cap = 0;
   FilterDev* d_filters = nullptr;
   size_t filt_cap = 0;
+  OrderDev* d_ord = nullptr;
+  size_t ord_cap = 0;
   float* d_means = nullptr;
   size_t means_cap = 0;
   WorkItem* d_items = nullptr;
@@ -121,6 +126,7 @@ struct Slot {
   float* d_u01 = nullptr;
   uint32_t* d_counter = nullptr;
   std::vector<FilterDev> h_filters;
+  std::vector<OrderDev> h_ord;
   std::vector<uint64_t> h_seedstep;
   std::vector<WorkItem> h_items;
   uint64_t evals = 0;
@@ -179,17 +185,19 @@ struct a3d_server {
 
 static int alloc_session(a3d_server* S, Session& se) {
   const size_t n = se.n;
-  se.bytes = 7 * n * 4 + scratch_floats((uint32_t)n) * 4 + n + 64;
+  se.bytes = 7 * n * 4 + scratch_floats((uint32_t)n) * 4 + n + 64 + order_stride((uint32_t)n);
   if (S->hbm_used + se.bytes > S->cfg.hbm_budget_bytes)
     return fail(A3D_ERR_NOMEM, "serving: session does not fit in the HBM budget");
   if (hipMalloc(&se.state, 7 * n * 4) != hipSuccess || hipMalloc(&se.scratch, scratch_floats((uint32_t)n) * 4) != hipSuccess ||
-      hipMalloc(&se.inmap, n) != hipSuccess || hipMalloc(&se.mean, 64) != hipSuccess) {
+      hipMalloc(&se.inmap, n) != hipSuccess || hipMalloc(&se.mean, 64) != hipSuccess ||
+      hipMalloc(&se.ord, order_stride((uint32_t)n)) != hipSuccess) {
     (void)hipFree(se.state);
     (void)hipFree(se.scratch);
     (void)hipFree(se.inmap);
     (void)hipFree(se.mean);
+    (void)hipFree(se.ord);
     se.state = se.scratch = se.mean = nullptr;
-    se.inmap = nullptr;
+    se.inmap = se.ord = nullptr;
     return fail(A3D_ERR_NOMEM, "serving: device allocation failed");
   }
   S->hbm_used += se.bytes;
@@ -201,8 +209,9 @@ static void free_session(a3d_server* S, Session& se) {
   (void)hipFree(se.scratch);
   (void)hipFree(se.inmap);
   (void)hipFree(se.mean);
+  (void)hipFree(se.ord);
   se.state = se.scratch = se.mean = nullptr;
-  se.inmap = nullptr;
+  se.inmap = se.ord = nullptr;
   S->hbm_used -= std::min(S->hbm_used, se.bytes);
   se.open = false;
 }
@@ -216,6 +225,7 @@ bool a3d_server::launch(Slot& s) {
   }
   if (!grow_pinned(s.h_data, s.h_cap, data + 16) || !grow_dev(s.d_data, s.d_cap, data + 16) ||
       !grow_dev(s.d_z4, s.z4_cap, z4 + 16) || !grow_dev(s.d_filters, s.filt_cap, nf) ||
+      !grow_dev(s.d_ord, s.ord_cap, nf) ||
       !grow_dev(s.d_seedstep, s.ss_cap, 2 * nf) || !grow_pinned(s.h_mean, s.h_mean_cap, 4 * nf)) {
     worker_error = "serving: staging allocation failed";
     return false;
@@ -225,9 +235,11 @@ bool a3d_server::launch(Slot& s) {
     return false;
   }
   s.h_filters.clear();
+  s.h_ord.clear();
   s.h_seedstep.clear();
   s.h_items.clear();
   const int ppw = clamp_ppw(current_ppw());
+  const bool spatial = current_order() == A3D_ORDER_SPATIAL;
   size_t off = 0, zoff = 0;
   uint32_t max_n = 0;
   s.evals = 0;
@@ -256,8 +268,10 @@ bool a3d_server::launch(Slot& s) {
     }
     d.z4 = s.d_z4 + zoff;
     d.no_resample = r.resample ? 0u : 1u;
+    d.order = spatial ? se.ov().order : nullptr;
     zoff += 4 * (size_t)se.n;
     s.h_filters.push_back(d);
+    s.h_ord.push_back(se.ov());
     s.h_seedstep.push_back(se.seed);
     max_n = std::max(max_n, se.n);
     for (uint32_t p0 = 0; p0 < se.n; p0 += ppw)
@@ -277,6 +291,7 @@ bool a3d_server::launch(Slot& s) {
   bool ok = hipMemcpyAsync(s.d_data, s.h_data, off * 4, hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(s.d_filters, s.h_filters.data(), nf * sizeof(FilterDev), hipMemcpyHostToDevice, st) ==
                 hipSuccess &&
+            hipMemcpyAsync(s.d_ord, s.h_ord.data(), nf * sizeof(OrderDev), hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(s.d_seedstep, s.h_seedstep.data(), 2 * nf * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(s.d_items, s.h_items.data(), s.h_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                            st) == hipSuccess;
@@ -297,6 +312,7 @@ bool a3d_server::launch(Slot& s) {
     gl.idx64 |= maps[sessions[r.session].map]->g.idx64;
     gl.clean &= maps[sessions[r.session].map]->g.clean;
   }
+  if (spatial) launch_order_many(s.d_filters, s.d_ord, (uint32_t)nf, max_n, st);  // after predict moved them
   launch_weight(gl, fs, s.d_items, (uint32_t)s.h_items.size(), s.d_counter, ppw, 1, st);
   launch_update_tail(maps[map0]->g, fs, st);
   // means to the host before resample (getMean is read after update, Node.cpp:167)
@@ -446,6 +462,7 @@ int a3d_server_destroy(a3d_server* S) {
     (void)hipFree(s.d_data);
     (void)hipFree(s.d_z4);
     (void)hipFree(s.d_filters);
+    (void)hipFree(s.d_ord);
     (void)hipFree(s.d_means);
     (void)hipFree(s.d_items);
     (void)hipFree(s.d_seedstep);
