@@ -191,6 +191,10 @@ void launch_draws(const Filters& fs, const uint64_t* d_seed, const uint64_t* d_s
 
 // Cloud-weight kernel tile sizes.
 int clamp_ppw(int ppw);
+// Particles per work item of a work list for the given mapping and ppw (the
+// particle-lane kernels take an item per workgroup, the point-lane ones an
+// item per wavefront).
+int item_particles(int mapping, int ppw);
 int current_ppw();
 int current_mapping();  // A3D_MAP_PARTICLE_LANES or A3D_MAP_POINT_LANES
 int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL

@@ -41,7 +41,8 @@ struct a3d_fleet {
   WorkItem* d_items = nullptr;
   size_t items_cap = 0;
   uint32_t nitems = 0;
-  int items_ppw = 32;  // particles per work item of the staged list
+  int items_ppw = 32;      // kernel shape the staged list was cut for
+  int items_mapping = 0;
   bool items_spatial = false;  // the staged filters visit particles in spatial order
   // particle visiting orders: one view per filter in one allocation
   uint8_t* ord = nullptr;
@@ -291,16 +292,18 @@ int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const 
   rc = upload_filters(fl);
   if (rc) return rc;
   // Work list: blocks of particles, filters in decreasing cost n*m.
-  const int ppw = clamp_ppw(current_ppw());
+  const int ppw = clamp_ppw(current_ppw()), mapping = current_mapping();
+  const uint32_t per_item = (uint32_t)item_particles(mapping, ppw);
   std::vector<uint32_t> order(fl->F);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     return (uint64_t)fl->n[a] * m[a] > (uint64_t)fl->n[b] * m[b];
   });
   std::vector<WorkItem> items;
-  items.reserve(fl->total / ppw + fl->F);
+  items.reserve(fl->total / per_item + fl->F);
   for (uint32_t f : order)
-    for (uint32_t p0 = 0; p0 < fl->n[f]; p0 += ppw) items.push_back({f, p0, std::min<uint32_t>(ppw, fl->n[f] - p0), 0});
+    for (uint32_t p0 = 0; p0 < fl->n[f]; p0 += per_item)
+      items.push_back({f, p0, std::min<uint32_t>(per_item, fl->n[f] - p0), 0});
   if (items.size() > fl->items_cap) {
     A3D_HIP(hipStreamSynchronize(fl->stream));
     (void)hipFree(fl->d_items);
@@ -315,6 +318,7 @@ int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const 
   A3D_HIP(hipStreamSynchronize(fl->stream));  // host vectors go out of scope
   fl->nitems = (uint32_t)items.size();
   fl->items_ppw = ppw;
+  fl->items_mapping = mapping;
   fl->st_m.assign(m, m + fl->F);
   fl->staged = true;
   return A3D_OK;
@@ -323,6 +327,8 @@ int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const 
 int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent) {
   if (!fl || !g) return fail(A3D_ERR_ARG, "a3d_fleet_update_staged: null argument");
   if (!fl->staged) return fail(A3D_ERR_STATE, "a3d_fleet_update_staged: call a3d_fleet_stage_update first");
+  if (persistent >= 0 && current_mapping() != fl->items_mapping)
+    return fail(A3D_ERR_STATE, "a3d_fleet_update_staged: the mapping changed since staging; stage again");
   auto ev = fl->timing.take();
   if (ev.first) (void)hipEventRecord(ev.first, fl->stream);
   if (persistent >= 0) {

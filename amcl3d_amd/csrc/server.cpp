@@ -239,6 +239,7 @@ bool a3d_server::launch(Slot& s) {
   s.h_seedstep.clear();
   s.h_items.clear();
   const int ppw = clamp_ppw(current_ppw());
+  const uint32_t per_item = (uint32_t)item_particles(current_mapping(), ppw);
   const bool spatial = current_order() == A3D_ORDER_SPATIAL;
   size_t off = 0, zoff = 0;
   uint32_t max_n = 0;
@@ -274,8 +275,8 @@ bool a3d_server::launch(Slot& s) {
     s.h_ord.push_back(se.ov());
     s.h_seedstep.push_back(se.seed);
     max_n = std::max(max_n, se.n);
-    for (uint32_t p0 = 0; p0 < se.n; p0 += ppw)
-      s.h_items.push_back({(uint32_t)i, p0, std::min<uint32_t>(ppw, se.n - p0), 0});
+    for (uint32_t p0 = 0; p0 < se.n; p0 += per_item)
+      s.h_items.push_back({(uint32_t)i, p0, std::min<uint32_t>(per_item, se.n - p0), 0});
     s.evals += (uint64_t)se.n * r.m;
   }
   for (size_t i = 0; i < nf; ++i) s.h_seedstep.push_back(sessions[s.reqs[i].session].step);

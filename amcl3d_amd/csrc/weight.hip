@@ -621,40 +621,46 @@ __global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, F
   weight_lanes<Q, IDX64, false, CLEAN, true>(g, f, p0, count, out, cloud);
 }
 
-// One work item; a filter that names its own grid (a batch over several
-// maps) is weighted on that grid, else on the launch's.
+// One work item of up to kBlockWaves * P particles of one filter, taken by
+// a whole workgroup: wavefront w holds particles p0 + w*P .. and the
+// wavefronts walk the filter's cloud together.  A filter that names its own
+// grid (a batch over several maps) is weighted on that grid, else on the
+// launch's.
 template <int Q, bool IDX64, bool CLEAN>
 __device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl) {
-  const uint32_t count = min(it.count, (uint32_t)(64 / Q));
+  constexpr uint32_t P = 64 / Q;
+  const uint32_t w = threadIdx.x >> 6, p0 = it.p0 + w * P;
+  const uint32_t count = (w * P < it.count) ? min(P, it.count - w * P) : 0u;
   if (f.gdev) {
     const GridDev gf = *f.gdev;
-    weight_lanes<Q, IDX64, true, CLEAN, false>(gf, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN, true>(gf, f, p0, count, nullptr, cl);
   } else {
-    weight_lanes<Q, IDX64, true, CLEAN, false>(g, f, it.p0, count, nullptr, cl);
+    weight_lanes<Q, IDX64, true, CLEAN, true>(g, f, p0, count, nullptr, cl);
   }
 }
 
 template <int Q, bool IDX64, bool CLEAN>
-__global__ __launch_bounds__(kThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs, const WorkItem* items,
-                                                               uint32_t nitems, uint32_t* counter, int persistent) {
-  __shared__ float4 cloud[kWaves * kCloudChunk];
-  float4* cl = cloud + (threadIdx.x >> 6) * kCloudChunk;
-  const int lane = lane_id();
-  uint32_t k = blockIdx.x * kWaves + (threadIdx.x >> 6);
+__global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs,
+                                                                    const WorkItem* items, uint32_t nitems,
+                                                                    uint32_t* counter, int persistent) {
+  __shared__ float4 cloud[2 * kCloudChunk];
+  __shared__ uint32_t taken;
   if (!persistent) {
-    if (k >= nitems) return;
-    const WorkItem it = items[k];
-    lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cl);
+    if (blockIdx.x >= nitems) return;
+    const WorkItem it = items[blockIdx.x];
+    lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cloud);
     return;
   }
   for (uint32_t r = 0; r < kXcds; ++r) {
     const uint32_t x = (blockIdx.x + r) % kXcds, beg = xcd_range(nitems, x), len = xcd_range(nitems, x + 1) - beg;
     for (;;) {
-      uint32_t t = 0;
-      if (lane == 0) t = atomicAdd(counter + x, 1u);
-      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-      if (t >= len) break;
-      lanes_item<Q, IDX64, CLEAN>(g, fs[items[beg + t].filter], items[beg + t], cl);
+      if (threadIdx.x == 0) taken = atomicAdd(counter + x, 1u);
+      __syncthreads();
+      const uint32_t t = taken;
+      __syncthreads();  // every wavefront has read it before the next take
+      if (t >= len) break;  // uniform over the workgroup
+      const WorkItem it = items[beg + t];
+      lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cloud);
     }
   }
 }
@@ -814,28 +820,37 @@ void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items,
       int dev = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false, false>, kThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, false, true>, kThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], lanes_fleet_kernel<Q, true, false>, kThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[3], lanes_fleet_kernel<Q, true, true>, kThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false, false>,
+                                                         kBlockThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, false, true>,
+                                                         kBlockThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], lanes_fleet_kernel<Q, true, false>,
+                                                         kBlockThreads, 0);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[3], lanes_fleet_kernel<Q, true, true>,
+                                                         kBlockThreads, 0);
       if (cus < 1) cus = 256;
     }
     blocks = (uint32_t)(cus * std::max(1, per_cu[k]));
-    const uint32_t need = (nitems + kWaves - 1) / kWaves;
-    if (blocks > need) blocks = need;
+    if (blocks > nitems) blocks = nitems;  // one item per workgroup
     if (blocks == 0) blocks = 1;
     zero_u32_kernel<<<1, 64, 0, s>>>(counter, kXcds);
   } else {
-    blocks = (nitems + kWaves - 1) / kWaves;
+    blocks = nitems;
   }
   if (blocks == 0) return;
-  if (k == 0) lanes_fleet_kernel<Q, false, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else if (k == 1) lanes_fleet_kernel<Q, false, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else if (k == 2) lanes_fleet_kernel<Q, true, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else lanes_fleet_kernel<Q, true, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  const dim3 b(blocks), t(kBlockThreads);
+  if (k == 0) lanes_fleet_kernel<Q, false, false><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else if (k == 1) lanes_fleet_kernel<Q, false, true><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else if (k == 2) lanes_fleet_kernel<Q, true, false><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
+  else lanes_fleet_kernel<Q, true, true><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
 }
 
 }  // namespace
+
+int item_particles(int mapping, int ppw) {
+  ppw = clamp_ppw(ppw);
+  return mapping == 0 ? ppw * kBlockWaves : ppw;
+}
 
 int clamp_ppw(int ppw) {
   if (ppw <= 16) return 16;
