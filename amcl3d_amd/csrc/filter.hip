@@ -151,11 +151,287 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
   chain_body<K, PFX>(n, src, dst, prefix, reinterpret_cast<float*>(lds4));
 }
 
-// The last chains of update: the four mean terms (workgroup row 0) and
-// resample's running sum c over the new weights (row 1) run side by side.
+// ----------------------------------------------- exact sums in parallel
+// The sums over non-negative terms (wtp, wtr, wt, resample's running sum c)
+// are computed in parallel with the sequential sum's exact result.  While
+// the running sum s stays in one float binade with spacing u, s = k*u and
+// fl(s + x) = (k + t)*u, where t is x/u rounded to the nearest integer such
+// that ties make k + t even: t depends on x and u only, except at exact ties
+// where it depends on the parity of k.  A tile of terms is therefore one
+// parallel scan of a two-state map (parity in -> increment, parity out).
+// Each step is checked, with a conservative test in double, to stay in the
+// binade (the exact sum strictly inside it); the first step that may leave it
+// is done as the reference's float add, and the scan restarts after it with
+// the new binade.  A sum of non-negative terms crosses each binade once, so
+// that happens a few dozen times per sum.  Runs of zeros at the start (s =
+// +0: the sums start at +0 and never reach -0) are skipped by a search for
+// the first non-zero term.  After kExactRounds restarts (terms of mixed sign
+// can cross binades often) or a non-finite s, the workgroup finishes the sum
+// serially from LDS, still in the reference's order.
+constexpr int kExactPer = 4;                                      // terms per thread per tile
+constexpr uint32_t kExactTile = kChainThreads * kExactPer;
+constexpr int kExactRounds = 256;
+
+struct ExactLds {
+  uint32_t d0[kChainThreads / 64], d1[kChainThreads / 64], pp[kChainThreads / 64];
+  uint32_t first[kChainThreads / 64];
+  float s;
+  float stage[kExactTile];
+};
+
+// parity in {0, 1} -> (increment of k mod 2^32, parity out); fn_then(a, b)
+// applies a, then b
+struct ParityFn {
+  uint32_t d0, d1, p0, p1;
+};
+
+__device__ inline ParityFn fn_then(const ParityFn& a, const ParityFn& b) {
+  ParityFn r;
+  r.d0 = a.d0 + (a.p0 ? b.d1 : b.d0);
+  r.p0 = a.p0 ? b.p1 : b.p0;
+  r.d1 = a.d1 + (a.p1 ? b.d1 : b.d0);
+  r.p1 = a.p1 ? b.p1 : b.p0;
+  return r;
+}
+
+__device__ inline ParityFn shfl_up_fn(const ParityFn& f, int d) {
+  ParityFn r;
+  r.d0 = (uint32_t)__shfl_up((int)f.d0, d);
+  r.d1 = (uint32_t)__shfl_up((int)f.d1, d);
+  r.p0 = (uint32_t)__shfl_up((int)f.p0, d);
+  r.p1 = (uint32_t)__shfl_up((int)f.p1, d);
+  return r;
+}
+
+// The composition of the maps of all threads before this one (thread order).
+__device__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const ParityFn id{0u, 0u, 0u, 1u};
+  ParityFn inc = f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const ParityFn o = shfl_up_fn(inc, d);
+    if (lane >= d) inc = fn_then(o, inc);
+  }
+  ParityFn ex = shfl_up_fn(inc, 1);
+  if (lane == 0) ex = id;
+  if (lane == 63) {
+    L.d0[wave] = inc.d0;
+    L.d1[wave] = inc.d1;
+    L.pp[wave] = inc.p0 | (inc.p1 << 1);
+  }
+  __syncthreads();
+  ParityFn w = id;
+  for (int v = 0; v < wave; ++v) w = fn_then(w, ParityFn{L.d0[v], L.d1[v], L.pp[v] & 1u, L.pp[v] >> 1});
+  __syncthreads();
+  return fn_then(w, ex);
+}
+
+__device__ uint32_t block_min(uint32_t v, ExactLds& L) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d));
+  if (lane == 0) L.first[wave] = v;
+  __syncthreads();
+  uint32_t m = L.first[0];
+  for (int w = 1; w < kChainThreads / 64; ++w) m = min(m, L.first[w]);
+  __syncthreads();
+  return m;
+}
+
+__device__ inline bool is_checkpoint(uint32_t i, uint32_t n) { return (i % kGroup) == kGroup - 1 || i == n - 1; }
+
+// The sequential float sum of x[0..n) from +0, exactly; with prefix, the
+// running sum after element i is stored at prefix[i / kGroup] at the end of
+// every group of kGroup elements and after the last element.  Called by every
+// thread of a kChainThreads workgroup; returns the sum in every thread.
+__device__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefix, ExactLds& L) {
+  const uint32_t tid = threadIdx.x;
+  float s = 0.0f;
+  uint32_t pos = 0;
+  int rounds = 0;
+  while (pos < n) {  // uniform over the workgroup
+    const uint32_t tend = min(n, pos + kExactTile);
+    const uint32_t j0 = pos + tid * kExactPer;
+    float xv[kExactPer];
+#pragma unroll
+    for (int e = 0; e < kExactPer; ++e) xv[e] = (j0 + e < tend) ? x[j0 + e] : 0.0f;
+    if (rounds >= kExactRounds || !isfinite(s)) {
+      // serial remainder: the tile through LDS, one lane adds in order
+#pragma unroll
+      for (int e = 0; e < kExactPer; ++e) L.stage[tid * kExactPer + e] = xv[e];
+      __syncthreads();
+      if (tid == 0) {
+        float a = s;
+        for (uint32_t i = pos; i < tend; ++i) {
+          a += L.stage[i - pos];
+          if (prefix && is_checkpoint(i, n)) prefix[i / kGroup] = a;
+        }
+        L.s = a;
+      }
+      __syncthreads();
+      s = L.s;
+      pos = tend;
+      __syncthreads();
+      continue;
+    }
+    if (s == 0.0f) {  // +0: the sum is the first non-zero term
+      uint32_t first = tend;
+#pragma unroll
+      for (int e = kExactPer - 1; e >= 0; --e)
+        if (j0 + e < tend && xv[e] != 0.0f) first = j0 + e;
+      first = block_min(first, L);
+      if (prefix) {
+#pragma unroll
+        for (int e = 0; e < kExactPer; ++e)
+          if (j0 + e < first && is_checkpoint(j0 + e, n)) prefix[(j0 + e) / kGroup] = 0.0f;
+      }
+      if (first < tend) {
+        s = x[first];  // 0 + x is x exactly
+        if (prefix && tid == 0 && is_checkpoint(first, n)) prefix[first / kGroup] = s;
+        pos = first + 1;
+        ++rounds;
+      } else {
+        pos = tend;
+      }
+      continue;
+    }
+    // the binade of s: s = k*u, and a step is regular when the exact s + x
+    // lies strictly inside (lo, hi) (or |s + x| < 2^-125 on the subnormal grid)
+    const float as = fabsf(s);
+    const bool sub = as < 0x1p-125f;
+    const int ex = sub ? -126 : (int)((__float_as_uint(as) >> 23) & 0xffu) - 127;
+    const double u = sub ? 0x1p-149 : ldexp(1.0, ex - 23), inv_u = sub ? 0x1p149 : ldexp(1.0, 23 - ex);
+    const double lo = ldexp(1.0, ex), hi = ldexp(1.0, ex + 1);
+    const bool neg = s < 0.0f;
+    const int k0 = (int)((double)s * inv_u);  // |k0| <= 2^24
+    ParityFn fe[kExactPer];
+    bool hard[kExactPer];
+    ParityFn mine{0u, 0u, 0u, 1u};
+#pragma unroll
+    for (int e = 0; e < kExactPer; ++e) {
+      const double r = (double)xv[e] * inv_u;  // exact: a power-of-two scale
+      hard[e] = !(fabs(r) < 0x1p25);           // leaves the binade whatever k is (or non-finite)
+      ParityFn f{0u, 0u, 0u, 1u};
+      if (!hard[e]) {
+        const double qd = floor(r), fr = r - qd;
+        const int q = (int)qd;
+        if (fr == 0.5) {  // tie: k + t even
+          f.d0 = (uint32_t)(q + (q & 1));
+          f.d1 = (uint32_t)(q + ((q + 1) & 1));
+          f.p0 = 0u;
+          f.p1 = 0u;
+        } else {
+          const int t = q + (fr > 0.5 ? 1 : 0);
+          f.d0 = f.d1 = (uint32_t)t;
+          f.p0 = (uint32_t)t & 1u;
+          f.p1 = ((uint32_t)t + 1u) & 1u;
+        }
+      }
+      fe[e] = f;
+      mine = fn_then(mine, f);
+    }
+    const ParityFn pre = block_exclusive_fn(mine, L);
+    // k as a 32-bit word: exact (|k| <= 2^25) up to the first irregular
+    // step; past it, wrapped garbage that is never used
+    const uint32_t p0 = (uint32_t)k0 & 1u;
+    uint32_t k = (uint32_t)k0 + (p0 ? pre.d1 : pre.d0);
+    uint32_t par = p0 ? pre.p1 : pre.p0;
+    uint32_t bad = tend;
+    uint32_t kv[kExactPer];
+#pragma unroll
+    for (int e = 0; e < kExactPer; ++e) {
+      kv[e] = k;
+      if (j0 + e >= tend || bad < tend) continue;
+      const uint32_t kn = k + (par ? fe[e].d1 : fe[e].d0);
+      const double vd = (double)(int)k * u + (double)xv[e];  // the exact s + x, rounded to double
+      bool reg;
+      if (sub)
+        reg = fabs(vd) < 0x1p-125 && kn != 0u;
+      else
+        reg = neg ? (vd < -lo && vd > -hi) : (vd > lo && vd < hi);
+      reg = !hard[e] && (reg || xv[e] == 0.0f);  // s + 0 = s
+      if (!reg) {
+        bad = j0 + e;
+        continue;
+      }
+      k = kn;
+      par = kn & 1u;
+      kv[e] = kn;
+    }
+    const uint32_t c = block_min(bad, L);
+    // the terms before c are exact: s_i = k_i * u
+    if (prefix) {
+#pragma unroll
+      for (int e = 0; e < kExactPer; ++e)
+        if (j0 + e < c && is_checkpoint(j0 + e, n)) prefix[(j0 + e) / kGroup] = (float)((double)(int)kv[e] * u);
+    }
+    if (c == pos) {
+      if (tid == 0) L.s = s;
+    } else if (c - 1 >= j0 && c - 1 < j0 + kExactPer) {
+      L.s = (float)((double)(int)kv[c - 1 - j0] * u);
+    }
+    __syncthreads();
+    const float sp = L.s;
+    __syncthreads();
+    if (c < tend) {
+      s = sp + x[c];  // the step that may leave the binade: the reference's float add
+      if (prefix && tid == 0 && is_checkpoint(c, n)) prefix[c / kGroup] = s;
+      pos = c + 1;
+      ++rounds;
+    } else {
+      s = sp;
+      pos = tend;
+    }
+  }
+  return s;
+}
+
+// Whether any of w[0..n) is negative or not finite: the running sum c is
+// then not monotone, and resample selects serially (the flag goes to
+// sums[4]).  Called by every thread of the workgroup.
+__device__ float weights_irregular(uint32_t n, const float* w, ExactLds& L) {
+  uint32_t bad = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) bad |= (w[i] < 0.0f || !isfinite(w[i])) ? 1u : 0u;
+  return block_min(bad ? 0u : 1u, L) == 0u ? 1.0f : 0.0f;
+}
+
+// wtp and wtr (grid row y = 0, 1), wt, or resample's c with its checkpoints,
+// by exact_sum: one workgroup per filter and sum.
+__global__ __launch_bounds__(kChainThreads) void exact_chain_kernel(Filters fs, int which) {
+  __shared__ ExactLds lds;
+  const FilterDev& f = fsel(fs, blockIdx.x);
+  if (which == kPrefixOnly && f.no_resample) return;
+  const uint32_t n = f.n;
+  float* sc = f.scratch;
+  const float* src;
+  float* dst;
+  float* prefix = nullptr;
+  if (which == kTailTotals) {
+    src = sc + (size_t)blockIdx.y * n;
+    dst = sums_of(f) + blockIdx.y;
+  } else if (which == kWeightTotal) {
+    src = f.w;
+    dst = sums_of(f) + 2;
+  } else {  // kPrefixOnly
+    src = f.w;
+    dst = sums_of(f) + 3;
+    prefix = sc + sc_c32(n);
+  }
+  const float v = exact_sum(n, src, prefix, lds);
+  const float irregular = prefix ? weights_irregular(n, src, lds) : 0.0f;
+  if (threadIdx.x == 0) {
+    *dst = v;
+    if (prefix) sums_of(f)[4] = irregular;
+  }
+}
+
+
+// The last chains of update: the four mean terms (workgroup row 0, terms of
+// either sign: a serial chain) and resample's running sum c over the new
+// weights (row 1, exact_sum) run side by side.
 __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) {
   __shared__ float4 lds4[ChainLds<4, -1>::kFloats / 4];
-  static_assert(ChainLds<1, 0>::kFloats <= ChainLds<4, -1>::kFloats, "LDS of the prefix chain");
+  static_assert(sizeof(ExactLds) <= sizeof(lds4), "LDS of the prefix sum");
   const FilterDev& f = fsel(fs, blockIdx.x);
   const uint32_t n = f.n;
   float* sc = f.scratch;
@@ -164,11 +440,16 @@ __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) 
     float* const dst[4] = {f.mean, f.mean + 1, f.mean + 2, f.mean + 3};
     chain_body<4, -1>(n, src, dst, nullptr, reinterpret_cast<float*>(lds4));
   } else {
-    const float* const src[1] = {f.w};
-    float* const dst[1] = {sums_of(f) + 3};
-    chain_body<1, 0>(n, src, dst, sc + sc_c32(n), reinterpret_cast<float*>(lds4));
+    ExactLds& L = *reinterpret_cast<ExactLds*>(lds4);
+    const float v = exact_sum(n, f.w, sc + sc_c32(n), L);
+    const float irregular = weights_irregular(n, f.w, L);
+    if (threadIdx.x == 0) {
+      sums_of(f)[3] = v;
+      sums_of(f)[4] = irregular;
+    }
   }
 }
+
 
 // ----------------------------------------------------------------- predict
 __global__ void predict_kernel(Filters fs) {
@@ -325,11 +606,47 @@ __device__ inline uint32_t first_not_below(const T* c, uint32_t n, U u) {
   return lo < n ? lo : n - 1;
 }
 
+__device__ inline void resample_gather(const FilterDev& f, uint32_t m, uint32_t i) {
+  const uint32_t n = f.n;
+  float* sc = f.scratch;
+  if (f.idx_out) f.idx_out[m] = i;
+  sc[4 * (size_t)n + m] = f.x[i];
+  sc[5 * (size_t)n + m] = f.y[i];
+  sc[6 * (size_t)n + m] = f.z[i];
+  sc[7 * (size_t)n + m] = f.a[i];
+  sc[8 * (size_t)n + m] = f.wp[i];
+  sc[9 * (size_t)n + m] = f.wr[i];
+}
+
+// The reference's selection loop itself (ParticleFilter.cpp:203-217), one
+// thread per filter, for weights whose running sum is not monotone (a
+// negative or non-finite weight, flagged in sums[4] with the running sum):
+// index i only moves forward, which the parallel search cannot express.
+__global__ void resample_select_serial_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.x);
+  const uint32_t n = f.n;
+  if (threadIdx.x != 0 || f.no_resample || sums_of(f)[4] == 0.0f) return;
+  const float u01 = fs.u01 ? fs.u01[blockIdx.x] : f.u01;
+  const float factor = 1.f / (float)n;
+  const float r = factor * u01;
+  float c = f.w[0];
+  uint32_t i = 0;
+  for (uint32_t m = 0; m < n; ++m) {
+    const float u = r + factor * (float)m;
+    while (u > c) {
+      if (++i >= n) break;  // the reference reads past the end here: clamped below
+      c += f.w[i];
+    }
+    resample_gather(f, m, min(i, n - 1));
+  }
+}
+
 __global__ void resample_select_kernel(Filters fs, int mode) {
   const FilterDev& f = fsel(fs, blockIdx.y);
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = f.n;
   if (m >= n || f.no_resample) return;
+  if (mode == A3D_RESAMPLE_REFERENCE && sums_of(f)[4] != 0.0f) return;  // the serial kernel selects
   float* sc = f.scratch;
   const float u01 = fs.u01 ? fs.u01[blockIdx.y] : f.u01;
   uint32_t i;
@@ -368,13 +685,7 @@ __global__ void resample_select_kernel(Filters fs, int mode) {
     const double u = ((double)u01 + (double)m) * (1.0 / (double)n);
     i = first_not_below(c64, n, u);
   }
-  if (f.idx_out) f.idx_out[m] = i;
-  sc[4 * (size_t)n + m] = f.x[i];
-  sc[5 * (size_t)n + m] = f.y[i];
-  sc[6 * (size_t)n + m] = f.z[i];
-  sc[7 * (size_t)n + m] = f.a[i];
-  sc[8 * (size_t)n + m] = f.wp[i];
-  sc[9 * (size_t)n + m] = f.wr[i];
+  resample_gather(f, m, i);
 }
 
 __global__ void resample_commit_kernel(Filters fs) {
@@ -416,7 +727,7 @@ void launch_predict(const Filters& fs, hipStream_t s) {
 void launch_init(const Filters& fs, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
   init_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
-  chain_kernel<1, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
+  exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
   init_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
   chain_kernel<4, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kInitMean);
 }
@@ -424,9 +735,9 @@ void launch_init(const Filters& fs, hipStream_t s) {
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
   tail_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(g, fs);
-  chain_kernel<2, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kTailTotals);
+  exact_chain_kernel<<<dim3(fs.count, 2), kChainThreads, 0, s>>>(fs, kTailTotals);
   tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
-  chain_kernel<1, -1><<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
+  exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
   tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
   // the mean and, in a second workgroup per filter, resample's running sum c
   mean_prefix_kernel<<<dim3(fs.count, 2), kChainThreads, 0, s>>>(fs);
@@ -435,11 +746,12 @@ void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
 void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
   if (mode == A3D_RESAMPLE_REFERENCE) {
-    if (!have_prefix) chain_kernel<1, 0><<<fs.count, kChainThreads, 0, s>>>(fs, kPrefixOnly);
+    if (!have_prefix) exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kPrefixOnly);
   } else {
     prefix64_kernel<<<fs.count, 1024, 0, s>>>(fs);
   }
   resample_select_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs, mode);
+  if (mode == A3D_RESAMPLE_REFERENCE) resample_select_serial_kernel<<<fs.count, 64, 0, s>>>(fs);
   resample_commit_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
 }
 

@@ -450,6 +450,63 @@ def test_large_filter_sampled(gpu, room):
     assert np.array_equal(idx, oidx)
 
 
+def _adversarial_weights(case, n, rng):
+    """Weight vectors that stress a parallel restatement of the float running
+    sum: binade crossings, exact ties, zero runs, subnormals, huge jumps, terms
+    of both signs."""
+    if case == "lognormal":
+        return np.exp(rng.uniform(np.log(1e-30), np.log(1e10), n)).astype(F32)
+    if case == "ties":  # 1 + k 2^-24 ... : adding 2^-24 to a sum in [1, 2) is an exact tie
+        w = np.full(n, 2.0 ** -24, F32)
+        w[0] = 1.0
+        w[1::7] = 3 * 2.0 ** -24
+        w[2::11] = 2.0 ** -23
+        return w
+    if case == "zero_runs":
+        w = rng.uniform(0, 1, n).astype(F32)
+        w[: n // 3] = 0.0
+        w[n // 2: n // 2 + 5000] = 0.0
+        w[n // 3 + 1::5] = -0.0
+        return w
+    if case == "subnormal":
+        w = (rng.uniform(0, 1, n) * 1e-41).astype(F32)
+        w[n // 2:] = rng.uniform(0, 1e-3, n - n // 2).astype(F32)
+        return w
+    if case == "jumps":
+        w = rng.uniform(0, 1e-6, n).astype(F32)
+        w[::997] = 1e6
+        w[min(5, n - 1)] = 3e30
+        return w
+    if case == "signed":
+        return rng.normal(0, 1, n).astype(F32)
+    if case == "powers":  # sums landing exactly on powers of two
+        w = np.full(n, 1.0, F32)
+        w[1::3] = 0.5
+        w[2::3] = 0.5
+        return w
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["lognormal", "ties", "zero_runs", "subnormal", "jumps", "signed", "powers"])
+def test_resample_running_sum_adversarial(gpu, case):
+    """The running sum c of resample (and through it every index) on weights
+    chosen to exercise each branch of the device's exact parallel sum."""
+    rng = np.random.default_rng(hash(case) % 2 ** 32)
+    for n in (1, 63, 64, 65, 4095, 4097, 100_000):
+        w = _adversarial_weights(case, n, rng)
+        p = np.zeros((n, 7), F32)
+        p[:, 0] = np.arange(n)
+        p[:, 4] = w
+        pf = gpu.ParticleFilter()
+        pf.set_particles(p)
+        for u01 in (0.0, 0.37, 0.999):
+            pf.set_particles(p)
+            idx = pf.resample(u01, gpu.RESAMPLE_REFERENCE, return_indices=True)
+            oidx = oracle.resample_indices(w, u01, 0)
+            assert np.array_equal(idx, oidx), (case, n, u01, int((idx != oidx).sum()))
+        pf.close()
+
+
 def test_graph_capture_replays_match_eager(gpu, room):
     pose, cloud, ranges, parts, z4, u01 = _full_step_case(gpu, room, 31)
     mods, deltas = [0.1, 0.1, 0.1, 0.3], [0.05, -0.03, 0.01, 0.02]
