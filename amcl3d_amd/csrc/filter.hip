@@ -425,31 +425,28 @@ __global__ __launch_bounds__(kChainThreads) void exact_chain_kernel(Filters fs, 
   }
 }
 
-
-// The last chains of update: the four mean terms (workgroup row 0, terms of
-// either sign: a serial chain) and resample's running sum c over the new
-// weights (row 1, exact_sum) run side by side.
+// The last sums of update, side by side: the four mean terms (grid rows
+// y = 0..3) and resample's running sum c over the new weights with its
+// checkpoints and the monotonicity flag (row 4).  The mean terms may have
+// either sign: exact_sum handles them exactly, restarting at every binade
+// crossing, and finishes serially if they cross too often.
 __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) {
-  __shared__ float4 lds4[ChainLds<4, -1>::kFloats / 4];
-  static_assert(sizeof(ExactLds) <= sizeof(lds4), "LDS of the prefix sum");
+  __shared__ ExactLds lds;
   const FilterDev& f = fsel(fs, blockIdx.x);
   const uint32_t n = f.n;
   float* sc = f.scratch;
-  if (blockIdx.y == 0) {
-    const float* const src[4] = {sc, sc + n, sc + 2 * (size_t)n, sc + 3 * (size_t)n};
-    float* const dst[4] = {f.mean, f.mean + 1, f.mean + 2, f.mean + 3};
-    chain_body<4, -1>(n, src, dst, nullptr, reinterpret_cast<float*>(lds4));
+  if (blockIdx.y < 4) {
+    const float v = exact_sum(n, sc + (size_t)blockIdx.y * n, nullptr, lds);
+    if (threadIdx.x == 0) f.mean[blockIdx.y] = v;
   } else {
-    ExactLds& L = *reinterpret_cast<ExactLds*>(lds4);
-    const float v = exact_sum(n, f.w, sc + sc_c32(n), L);
-    const float irregular = weights_irregular(n, f.w, L);
+    const float v = exact_sum(n, f.w, sc + sc_c32(n), lds);
+    const float irregular = weights_irregular(n, f.w, lds);
     if (threadIdx.x == 0) {
       sums_of(f)[3] = v;
       sums_of(f)[4] = irregular;
     }
   }
 }
-
 
 // ----------------------------------------------------------------- predict
 __global__ void predict_kernel(Filters fs) {
@@ -739,8 +736,8 @@ void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
   tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
   exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kWeightTotal);
   tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
-  // the mean and, in a second workgroup per filter, resample's running sum c
-  mean_prefix_kernel<<<dim3(fs.count, 2), kChainThreads, 0, s>>>(fs);
+  // the mean and, beside it, resample's running sum c
+  mean_prefix_kernel<<<dim3(fs.count, 5), kChainThreads, 0, s>>>(fs);
 }
 
 void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s) {

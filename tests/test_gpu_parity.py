@@ -380,6 +380,29 @@ def test_full_step_bit_exact(gpu, room, seed):
     _say("seed", seed, "resample indices differing between float and FP64 prefix:", int((ref_idx != f64_idx).sum()))
 
 
+@pytest.mark.parametrize("n", [3000, 30000])
+def test_update_mean_of_mixed_sign_terms(gpu, room, n):
+    """Yaw spread over (-pi, pi]: the mean's yaw sum has terms of both signs,
+    whose running sum crosses float binades again and again (the device's
+    exact parallel sum restarts at each crossing and goes serial after many);
+    the mean must still be the reference's sequential float sum."""
+    rng = np.random.default_rng(41 + n)
+    bm = room["map"]
+    pose = np.append(W.free_pose(bm, rng, z_range=(1.0, 2.5)), 0.0)
+    cloud = W.sensor_cloud(bm, pose, 1500, 0.05, rng)
+    ranges = W.beacons(bm, pose, 2, 0.1, rng)
+    parts = W.particles_around(pose, n, rng, dxyz=3.0)
+    parts[:, 3] = rng.uniform(-np.pi, np.pi, n).astype(F32)
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    pf.update(room["grid"], cloud, ranges, 0.8, 0.1, 0.0, 0.0)
+    op, om = oracle.pf_update(parts, room["ogrid"], cloud, ranges, 0.8, 0.1, 0.0, 0.0)
+    assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32))
+    assert np.array_equal(pf.getMean().view(np.int32), om.view(np.int32))
+    idx = pf.resample(0.61, gpu.RESAMPLE_REFERENCE, return_indices=True)
+    assert np.array_equal(idx, oracle.resample_indices(op[:, 4], 0.61, 0))
+
+
 @pytest.mark.parametrize("mapping,ppw", [(0, 64), (0, 16), (1, 32)])
 def test_update_visiting_order_invariance(gpu, room, mapping, ppw):
     """The particle visiting order (index or spatial) and the kernel shape
