@@ -44,16 +44,17 @@ struct ChainLds {
   static constexpr int kFloats = 2 * K * kRow;
 };
 
-// Sequential float sums src[k][0..n) -> *dst[k] for k < K (lane k of wave 0
-// runs chain k); with PFX >= 0, chain PFX's running sum at the end of every
-// group of kGroup elements goes to prefix[0 .. ceil(n / kGroup)).
+// Sequential float sums acc0 + src[k][0] + ... + src[k][n-1] -> *dst[k] for
+// k < K (lane k of wave 0 runs chain k); with PFX >= 0, chain PFX's running
+// sum at the end of every group of kGroup elements goes to prefix[0 ..
+// ceil(n / kGroup)).  Every reference sum starts from 0 (0 + x is x exactly).
 template <int K, int PFX>
-__device__ void chain_body(uint32_t n, const float* const (&src)[K], float* const (&dst)[K], float* prefix,
-                           float* lds) {
+__device__ __forceinline__ void chain_body(uint32_t n, const float* const (&src)[K], float* const (&dst)[K],
+                                           float* prefix, float* lds, float acc0 = 0.f) {
   float* stage = lds;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, nthr = blockDim.x;
   const uint32_t nblk = (n + kCH - 1) / kCH;
-  float acc = 0.f;  // every reference sum starts from 0 (0 + x is x exactly)
+  float acc = acc0;
   for (int e = tid; e < K * kCH; e += nthr) {
     const int k = e / kCH, j = e % kCH;
     stage[k * kRow + j] = ((uint32_t)j < n) ? src[k][j] : 0.f;
@@ -163,14 +164,23 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
 // binade (the exact sum strictly inside it); the first step that may leave it
 // is done as the reference's float add, and the scan restarts after it with
 // the new binade.  A sum of non-negative terms crosses each binade once, so
-// that happens a few dozen times per sum.  Runs of zeros at the start (s =
-// +0: the sums start at +0 and never reach -0) are skipped by a search for
-// the first non-zero term.  After kExactRounds restarts (terms of mixed sign
-// can cross binades often) or a non-finite s, the workgroup finishes the sum
-// serially from LDS, still in the reference's order.
-constexpr int kExactPer = 4;                                      // terms per thread per tile
+// that happens a few dozen times per sum, mostly within its first terms,
+// which are added serially.  Runs of zeros (s = +0: the sums start at +0 and
+// never reach -0) are skipped by a search for the first non-zero term.  When
+// restarts come densely (terms of mixed sign can cross binades all the time)
+// or s is not finite, the workgroup finishes the sum serially from LDS,
+// still in the reference's order.
+constexpr int kExactPer = 8;                                      // terms per thread per tile
 constexpr uint32_t kExactTile = kChainThreads * kExactPer;
-constexpr int kExactRounds = 256;
+// Restarts that each advanced fewer than kExactDenseRun terms: after
+// kExactDense of them in a row (terms of mixed sign crossing binades all the
+// time) the rest of the sum is added serially.
+constexpr int kExactDense = 4;
+constexpr uint32_t kExactDenseRun = 1024;
+// A growing sum crosses a binade each time it doubles, mostly within its
+// first terms: those are added serially, one restart per crossing would cost
+// more.
+constexpr uint32_t kExactHead = 1024;
 
 struct ExactLds {
   uint32_t d0[kChainThreads / 64], d1[kChainThreads / 64], pp[kChainThreads / 64];
@@ -204,7 +214,7 @@ __device__ inline ParityFn shfl_up_fn(const ParityFn& f, int d) {
 }
 
 // The composition of the maps of all threads before this one (thread order).
-__device__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
+__device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const ParityFn id{0u, 0u, 0u, 1u};
   ParityFn inc = f;
@@ -221,13 +231,23 @@ __device__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
     L.pp[wave] = inc.p0 | (inc.p1 << 1);
   }
   __syncthreads();
-  ParityFn w = id;
-  for (int v = 0; v < wave; ++v) w = fn_then(w, ParityFn{L.d0[v], L.d1[v], L.pp[v] & 1u, L.pp[v] >> 1});
+  constexpr int kW = kChainThreads / 64;
+  uint32_t d0[kW], d1[kW], pp[kW];  // all LDS reads first
+#pragma unroll
+  for (int v = 0; v < kW; ++v) {
+    d0[v] = L.d0[v];
+    d1[v] = L.d1[v];
+    pp[v] = L.pp[v];
+  }
   __syncthreads();
+  ParityFn w = id;
+#pragma unroll
+  for (int v = 0; v < kW - 1; ++v)
+    if (v < wave) w = fn_then(w, ParityFn{d0[v], d1[v], pp[v] & 1u, pp[v] >> 1});
   return fn_then(w, ex);
 }
 
-__device__ uint32_t block_min(uint32_t v, ExactLds& L) {
+__device__ __forceinline__ uint32_t block_min(uint32_t v, ExactLds& L) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d));
   if (lane == 0) L.first[wave] = v;
@@ -240,38 +260,84 @@ __device__ uint32_t block_min(uint32_t v, ExactLds& L) {
 
 __device__ inline bool is_checkpoint(uint32_t i, uint32_t n) { return (i % kGroup) == kGroup - 1 || i == n - 1; }
 
+// a + stage[0] + stage[1] + ... in order (one lane), for the terms of global
+// indices base .. base + len - 1, with their checkpoints.  The terms go in
+// groups of kGroup aligned to the checkpoint groups, each group's LDS reads
+// issued together, so the adds wait on each other only.
+__device__ __forceinline__ float serial_add(float a, const float* stage, uint32_t base, uint32_t len, uint32_t n, float* prefix) {
+  uint32_t j = 0;
+  const uint32_t lead = min(len, (kGroup - base % kGroup) % kGroup);
+  for (; j < lead; ++j) {
+    a += stage[j];
+    if (prefix && is_checkpoint(base + j, n)) prefix[(base + j) / kGroup] = a;
+  }
+  for (; j + kGroup <= len; j += kGroup) {
+    float v[kGroup];
+#pragma unroll
+    for (uint32_t u = 0; u < kGroup; ++u) v[u] = stage[j + u];
+#pragma unroll
+    for (uint32_t u = 0; u < kGroup; ++u) a += v[u];
+    if (prefix) prefix[(base + j) / kGroup] = a;  // base + j + kGroup - 1 ends a group
+  }
+  for (; j < len; ++j) {
+    a += stage[j];
+    if (prefix && is_checkpoint(base + j, n)) prefix[(base + j) / kGroup] = a;
+  }
+  return a;
+}
+
+// s + x[pos] + ... + x[n-1] in order: lane 0 adds singly up to the next
+// checkpoint-group boundary, then the serial chain (chain_body) runs the rest.
+__device__ __forceinline__ float serial_rest(float s, uint32_t pos, uint32_t n, const float* __restrict__ x,
+                                             float* prefix, ExactLds& L) {
+  static_assert(ChainLds<1, 0>::kFloats <= kExactTile, "the chain's staging fits the tile buffer");
+  const uint32_t pa = min(n, (pos + kGroup - 1) / kGroup * kGroup);
+  if (threadIdx.x == 0) L.s = serial_add(s, x + pos, pos, pa - pos, n, prefix);
+  __syncthreads();
+  s = L.s;
+  __syncthreads();
+  if (pa < n) {
+    const float* const src[1] = {x + pa};
+    float* const dst[1] = {&L.s};
+    if (prefix)
+      chain_body<1, 0>(n - pa, src, dst, prefix + pa / kGroup, L.stage, s);
+    else
+      chain_body<1, -1>(n - pa, src, dst, nullptr, L.stage, s);
+    __syncthreads();
+    s = L.s;
+    __syncthreads();
+  }
+  return s;
+}
+
 // The sequential float sum of x[0..n) from +0, exactly; with prefix, the
 // running sum after element i is stored at prefix[i / kGroup] at the end of
 // every group of kGroup elements and after the last element.  Called by every
 // thread of a kChainThreads workgroup; returns the sum in every thread.
-__device__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefix, ExactLds& L) {
+__device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefix, ExactLds& L) {
   const uint32_t tid = threadIdx.x;
   float s = 0.0f;
   uint32_t pos = 0;
-  int rounds = 0;
+  int dense = 0;
+  {  // the serial head
+    const uint32_t hend = min(n, kExactHead);
+    for (uint32_t i = tid; i < hend; i += blockDim.x) L.stage[i] = x[i];
+    __syncthreads();
+    if (tid == 0) L.s = serial_add(0.0f, L.stage, 0, hend, n, prefix);
+    __syncthreads();
+    s = L.s;
+    pos = hend;
+    __syncthreads();
+  }
   while (pos < n) {  // uniform over the workgroup
     const uint32_t tend = min(n, pos + kExactTile);
     const uint32_t j0 = pos + tid * kExactPer;
     float xv[kExactPer];
 #pragma unroll
     for (int e = 0; e < kExactPer; ++e) xv[e] = (j0 + e < tend) ? x[j0 + e] : 0.0f;
-    if (rounds >= kExactRounds || !isfinite(s)) {
-      // serial remainder: the tile through LDS, one lane adds in order
-#pragma unroll
-      for (int e = 0; e < kExactPer; ++e) L.stage[tid * kExactPer + e] = xv[e];
-      __syncthreads();
-      if (tid == 0) {
-        float a = s;
-        for (uint32_t i = pos; i < tend; ++i) {
-          a += L.stage[i - pos];
-          if (prefix && is_checkpoint(i, n)) prefix[i / kGroup] = a;
-        }
-        L.s = a;
-      }
-      __syncthreads();
-      s = L.s;
-      pos = tend;
-      __syncthreads();
+    if (dense >= kExactDense || !isfinite(s)) {
+      s = serial_rest(s, pos, n, x, prefix, L);
+      pos = n;
       continue;
     }
     if (s == 0.0f) {  // +0: the sum is the first non-zero term
@@ -289,7 +355,6 @@ __device__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefi
         s = x[first];  // 0 + x is x exactly
         if (prefix && tid == 0 && is_checkpoint(first, n)) prefix[first / kGroup] = s;
         pos = first + 1;
-        ++rounds;
       } else {
         pos = tend;
       }
@@ -376,8 +441,8 @@ __device__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefi
     if (c < tend) {
       s = sp + x[c];  // the step that may leave the binade: the reference's float add
       if (prefix && tid == 0 && is_checkpoint(c, n)) prefix[c / kGroup] = s;
+      dense = (c - pos < kExactDenseRun) ? dense + 1 : 0;
       pos = c + 1;
-      ++rounds;
     } else {
       s = sp;
       pos = tend;
@@ -389,7 +454,7 @@ __device__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefi
 // Whether any of w[0..n) is negative or not finite: the running sum c is
 // then not monotone, and resample selects serially (the flag goes to
 // sums[4]).  Called by every thread of the workgroup.
-__device__ float weights_irregular(uint32_t n, const float* w, ExactLds& L) {
+__device__ __forceinline__ float weights_irregular(uint32_t n, const float* w, ExactLds& L) {
   uint32_t bad = 0;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) bad |= (w[i] < 0.0f || !isfinite(w[i])) ? 1u : 0u;
   return block_min(bad ? 0u : 1u, L) == 0u ? 1.0f : 0.0f;
