@@ -77,6 +77,11 @@ SIGNATURES = {
     "a3d_grid_read_cells": [_vp, _u64, _u64, _vp, _vp],
     "a3d_grid_is_into_map": [_vp, _f, _f, _f, _p(_i)],
     "a3d_grid_cloud_weight": [_vp, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _sz, _f, _f, _vp],
+    "a3d_octree_read": [ctypes.c_char_p, _p(_vp)],
+    "a3d_octree_get_info": [_vp, _vp, _vp, _p(_d), _p(_u64), _p(_u64)],
+    "a3d_octree_get_points": [_vp, _u64, _u64, _vp],
+    "a3d_octree_destroy": [_vp],
+    "a3d_grid_open": [ctypes.c_char_p, _d, _i, _p(_vp)],
     "a3d_voxel_grid": [_vp, _sz, _sz, _f, _vp, _vp, _vp, _p(_sz)],
     "a3d_voxel_grid_dev": [_vp, _sz, _sz, _f, _vp, _vp, _vp, _p(_sz)],
     "a3d_pf_create": [_p(_vp)],

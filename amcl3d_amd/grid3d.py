@@ -1,12 +1,13 @@
 """Grid3d: the device-resident likelihood grid.
 
 Mirrors ``amcl3d::Grid3d`` (amcl3d/src/Grid3d.h:34-159) on top of the C ABI.
-``open`` builds the grid on the GPU with computeGrid
-(amcl3d/src/PointCloudTools.cpp:84-149) from the map's occupied points; the
-octomap file reader of the reference is out of scope, so the caller passes the
-points and bounds that ``openOcTree``/``computePointCloud`` would produce.
+``open(map_path, sensor_dev)`` is the reference's: the OcTree file (.bt/.ot),
+then its .grid cache or computeGrid (amcl3d/src/PointCloudTools.cpp:84-149) on
+the GPU.  ``open(points, min, max, res, sensor_dev)`` builds the grid from a
+map already read (the occupied points and bounds computePointCloud gives).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -21,12 +22,34 @@ class Grid3d:
         self.max = None
         self.res = None
         self.sensor_dev = None
+        self.last_error = None
 
     # -- construction ------------------------------------------------------
-    def open(self, points, min_xyz, max_xyz, res, sensor_dev, keep_dist=False):
-        """Grid3d::open with the octomap already read (Grid3d.cpp:25-78):
-        computeGrid on the device.  Returns True on success, like the reference."""
+    def open(self, points, min_xyz=None, max_xyz=None, res=None, sensor_dev=None, keep_dist=False):
+        """Grid3d::open (Grid3d.cpp:25-78).
+
+        ``open(map_path, sensor_dev)``: the OcTree file, the .grid cache beside
+        it, else computeGrid on the device and the cache written.  Returns
+        False when the file cannot be used (the reason is in ``last_error``),
+        like the reference; device failures raise.
+
+        ``open(points, min, max, res, sensor_dev)``: computeGrid on the device
+        from the map's occupied points.  Returns True."""
         self.close()
+        if isinstance(points, (str, bytes, os.PathLike)):
+            sd = min_xyz if sensor_dev is None else sensor_dev
+            h = ctypes.c_void_p()
+            path = os.fsencode(points)
+            try:
+                _lib.call("a3d_grid_open", path, float(sd), 1 if keep_dist else 0, ctypes.byref(h))
+            except _lib.Amcl3dError as e:
+                if e.code != _lib.A3D_ERR_IO:
+                    raise
+                self.last_error = str(e)
+                return False
+            self._h = h.value
+            self._read_info()
+            return True
         pts = _lib.f32(points).reshape(-1, 3)
         h = ctypes.c_void_p()
         _lib.call("a3d_grid_create_from_points", _lib.ptr(pts) if pts.shape[0] else None, pts.shape[0],

@@ -1,11 +1,13 @@
-"""Profiling driver: the cloud-weight kernel of the large-filter workload,
-then a device copy of known size.
+"""Profiling driver: the cloud-weight kernel of the large-filter workload (or,
+with ``--fleet``, the persistent batched launch of a 256-filter fleet), then a
+device copy of known size.
 
 Under ``rocprofv3 --pmc ...`` the copy kernel's FETCH_SIZE / WRITE_SIZE
 (it reads and writes ``--copy-mib`` MiB) calibrate the counters that the
 cloud-weight dispatches report.
 
     python -m amcl3d_amd.profile_run --updates 3 --copy-mib 1024 --shape 0,16,1
+    python -m amcl3d_amd.profile_run --fleet building --updates 3 --shape 0,16,1
 """
 import argparse
 import sys
@@ -22,6 +24,8 @@ def main(argv=None):
     ap.add_argument("--points", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--shape", default="", help="cloud-weight kernel shape 'MAPPING,PPW,ORDER' (default: the library's)")
+    ap.add_argument("--fleet", choices=("room", "building"), default="",
+                    help="profile the batched launch of 256 filters on this map instead")
     args = ap.parse_args(argv)
 
     import amcl3d_amd
@@ -33,6 +37,45 @@ def main(argv=None):
 
         autotune.set_shape(tuple(int(v) for v in args.shape.split(",")))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.fleet:
+        run_fleet(amcl3d_amd, benchlib, W, args, log)
+    else:
+        run_large_filter(amcl3d_amd, benchlib, W, args, log)
+    nbytes = args.copy_mib << 20
+    a = amcl3d_amd.DeviceBuffer(nbytes)
+    b = amcl3d_amd.DeviceBuffer(nbytes)
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _lib.call("a3d_buffer_copy", b.ptr, a.ptr, nbytes)
+        log("copy %d MiB: %.3f ms" % (args.copy_mib, (time.perf_counter() - t0) * 1e3))
+    print("profile_run done", flush=True)
+
+
+def run_fleet(amcl3d_amd, benchlib, W, args, log):
+    """The batched workload: 256 filters (N log-uniform 500..50k, M 1k..30k)
+    on the room map or on the building map, one persistent launch per update."""
+    from amcl3d_amd import fleet as fleet_mod
+
+    if args.fleet == "room":
+        bm, grid = benchlib.room_grid(amcl3d_amd, W)
+    else:
+        bm = W.building_map(args.seed)
+        grid = amcl3d_amd.Grid3d()
+        grid.open(bm.points(dedup=False), bm.min, bm.max, bm.res, 0.05)
+    counts, parts, clouds, ranges = benchlib.fleet_case(W, bm, 256, 3)
+    fl = fleet_mod.Fleet(counts)
+    fl.set_particles(np.concatenate(parts))
+    fl.stage_update(clouds, ranges, 0.8, 0.1)
+    for _ in range(args.updates):
+        fl.update_staged(grid, fleet_mod.PERSISTENT)
+        fl.synchronize()
+    ms, n, ev = fl.timing()
+    log("fleet cloud weight on the %s map: %d launches, %.3f ms average, %.3e evals/s"
+        % (args.fleet, n, ms, ev / n / (ms * 1e-3)))
+    fl.close()
+
+
+def run_large_filter(amcl3d_amd, benchlib, W, args, log):
     wl = benchlib.large_filter_workload(amcl3d_amd, W, args.updates, args.seed, args.particles, args.points, 8,
                                         log=log)
     pf = amcl3d_amd.ParticleFilter()
@@ -49,14 +92,6 @@ def main(argv=None):
         pf.synchronize()
     ms, n, ev = pf.timing()
     log("cloud weight: %d launches, %.3f ms average, %.3e evals/s" % (n, ms, ev / n / (ms * 1e-3)))
-    nbytes = args.copy_mib << 20
-    a = amcl3d_amd.DeviceBuffer(nbytes)
-    b = amcl3d_amd.DeviceBuffer(nbytes)
-    for _ in range(3):
-        t0 = time.perf_counter()
-        _lib.call("a3d_buffer_copy", b.ptr, a.ptr, nbytes)
-        log("copy %d MiB: %.3f ms" % (args.copy_mib, (time.perf_counter() - t0) * 1e3))
-    print("profile_run done", flush=True)
 
 
 if __name__ == "__main__":

@@ -114,6 +114,33 @@ int a3d_grid_cloud_weight(const a3d_grid* g, const float* x, const float* y, con
                           const float* tx, const float* ty, const float* tz, const float* yaw, size_t n, float roll,
                           float pitch, float* out);
 
+/* ------------------------------------------------------------ OcTree maps */
+/* openOcTree + computePointCloud (PointCloudTools.cpp:26-82) on the host:
+ * reads an octomap OcTree file, binary (.bt, OcTree::readBinary) or full
+ * (.ot, AbstractOcTree::read, tree type OcTree), and keeps the centres of its
+ * occupied leaves as floats, in the reference's leaf-iterator order (depth
+ * first, children in octomap's order), with the metric bounds of all leaves
+ * (getMetricMin/Max) and the resolution.  Errors (A3D_ERR_IO) carry the
+ * reference's messages: "Cannot find file <path>", "OcTree cannot be read"
+ * (.bt), "OcTree cannot be created from file <path>" (.ot, other extensions).
+ * Truncated data and nodes below depth 16 are errors. */
+typedef struct a3d_octree a3d_octree;
+int a3d_octree_read(const char* path, a3d_octree** out);
+/* nodes is OcTree::size(); occupied the number of occupied leaf centres. */
+int a3d_octree_get_info(const a3d_octree* t, double min[3], double max[3], double* res, uint64_t* nodes,
+                        uint64_t* occupied);
+/* Centres [first, first+count) as x, y, z floats (host memory). */
+int a3d_octree_get_points(const a3d_octree* t, uint64_t first, uint64_t count, float* xyz);
+int a3d_octree_destroy(a3d_octree* t);
+/* Grid3d::open(map_path, sensor_dev) (Grid3d.cpp:25-78): the OcTree file
+ * (A3D_ERR_IO "OcTree is empty" when it has at most one node), then the .grid
+ * cache named like the reference's (the map path up to the first ".bt" /
+ * ".ot" in it, plus ".grid") when it loads with this sensor_dev and the map's
+ * sizes, else computeGrid on the device and the cache written (a failed write
+ * is ignored, as in the reference).  keep_dist as in
+ * a3d_grid_create_from_points. */
+int a3d_grid_open(const char* map_path, double sensor_dev, int keep_dist, a3d_grid** out);
+
 /* ------------------------------------------------------------ VoxelGrid */
 /* pcl::VoxelGrid<pcl::PointXYZ> as the node applies it to each scan before
  * update (Node.cpp:130-141: setLeafSize(voxel_size) on all axes, PCL

@@ -100,6 +100,12 @@ class Grid3d {
   Grid3d& operator=(const Grid3d&) = delete;
   ~Grid3d() { close(); }
 
+  // Grid3d::open(map_path, sensor_dev) (Grid3d.cpp:25-78): the OcTree file
+  // (.bt / .ot), its .grid cache, else computeGrid on the device.
+  bool open(const std::string& map_path, double sensor_dev) {
+    close();
+    return a3d_grid_open(map_path.c_str(), sensor_dev, 0, &g_) == A3D_OK;
+  }
   // Builds the grid on the device from the occupied points (computeGrid).
   bool open(const float* xyz, size_t n, const double min[3], const double max[3], double res, double sensor_dev) {
     close();
