@@ -7,6 +7,9 @@ update versus the plain single-filter kernel launched once per filter.  The
 same fleet on the large filter's building map (2000 x 2000 x 600 cells, where
 spread particle sets make the gathers HBM-bound) is reported beside it.
 
+map_file: the building map stored as an OcTree .bt file (one occupied leaf
+per occupied cell) and read back the way Grid3d::open reads it.
+
 served: sessions on resident maps, each sending Poisson update requests at
 10 Hz; per session N log-uniform in 1,000..50,000, M uniform in
 2,000..20,000.  p50/p99 latency is request submit -> mean on the host.
@@ -226,7 +229,51 @@ def serving_maps(amcl3d_amd, W, building=None, log=print):
     return maps
 
 
-def extras(amcl3d_amd, W, plain_value=None, building=None, log=print):
+def map_file(amcl3d_amd, bm, sensor_dev=0.05, log=print):
+    """The map as an OcTree file (a .bt of its occupied cells), read back as
+    Grid3d::open reads it: openOcTree + computePointCloud on the host, then
+    computeGrid on the device from the points read.  Writing the file is set
+    up, not timed; the .grid cache Grid3d::open writes (8 bytes per cell) is
+    left out."""
+    import os
+    import shutil
+    import tempfile
+
+    from . import octree
+
+    d = tempfile.mkdtemp(prefix="a3d_map_")
+    path = os.path.join(d, "building.bt")
+    try:
+        t0 = time.time()
+        pts = bm.points(dedup=False)
+        nodes = octree.write_points(path, pts, bm.res)
+        del pts
+        write_s = time.time() - t0
+        nbytes = os.path.getsize(path)
+        t1 = time.perf_counter()
+        tree = octree.openOcTree(path)
+        info = octree.computePointCloud(tree)
+        read_s = time.perf_counter() - t1
+        tree.close()
+        g = amcl3d_amd.Grid3d()
+        amcl3d_amd.synchronize()
+        t2 = time.perf_counter()
+        g.open(info.cloud, info.min, info.max, info.octo_resol, sensor_dev)
+        amcl3d_amd.synchronize()
+        build_s = time.perf_counter() - t2
+        out = {"workload": "the 100x100x30 m building at 0.05 m as a .bt (%d nodes, %.1f MB): openOcTree + "
+                           "computePointCloud on the host, computeGrid on the device" % (nodes, nbytes / 1e6),
+               "occupied_points": int(info.cloud.shape[0]), "file_bytes": nbytes, "read_s": read_s,
+               "read_MB_per_s": nbytes / read_s / 1e6, "read_points_per_s": info.cloud.shape[0] / read_s,
+               "grid_cells": g.cells, "device_build_s": build_s, "open_without_cache_write_s": read_s + build_s,
+               "file_write_s_untimed": write_s}
+        g.close()
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def extras(amcl3d_amd, W, plain_value=None, building=None, building_map=None, log=print):
     bm, grid = room_grid(amcl3d_amd, W)
     out = {"batched": batched(amcl3d_amd, W, bm, grid, map_name="the 200x200x60 room map")}
     log("batched:", out["batched"]["batched_evals_per_s"], "evals/s")
@@ -246,4 +293,7 @@ def extras(amcl3d_amd, W, plain_value=None, building=None, log=print):
     if plain_value:
         out["served_over_plain_large_filter"] = out["served"]["served_evals_per_s"] / plain_value
     out["served_over_plain_loop"] = out["served"]["served_evals_per_s"] / out["batched"]["plain_loop_evals_per_s"]
+    if building_map is not None:
+        out["map_file"] = map_file(amcl3d_amd, building_map, log=log)
+        log("map file:", out["map_file"])
     return out

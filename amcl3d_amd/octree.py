@@ -237,6 +237,53 @@ def write_octree(path, depth, idx, occupied, res, values=None, check=True):
     return len(codes)
 
 
+def _spread3(v):
+    """The 16 bits of v moved to bits 0, 3, 6, ... 45."""
+    v = v.astype(np.uint64) & np.uint64(0xFFFF)
+    for sh, m in ((16, 0x0000FF0000FF), (8, 0x00F00F00F00F), (4, 0x0C30C30C30C3), (2, 0x249249249249)):
+        v = (v | (v << np.uint64(sh))) & np.uint64(m)
+    return v
+
+
+def _write_bt_cells(path, key, res):
+    """.bt of occupied depth-16 leaves at the distinct keys (n, 3) in one
+    pass over their sorted child-digit codes: leaf i opens the inner nodes
+    whose prefixes differ from leaf i-1's, at depths lcp .. 15, so the nodes
+    come out in pre-order without a sort of the inner nodes."""
+    code = np.unique(_spread3(key[:, 0]) | (_spread3(key[:, 1]) << np.uint64(1)) |
+                     (_spread3(key[:, 2]) << np.uint64(2)))
+    n = len(code)
+    # first level whose digit differs from the previous leaf (1..16)
+    diff = np.empty(n, np.uint64)
+    diff[0] = 0
+    np.bitwise_xor(code[1:], code[:-1], out=diff[1:])
+    _, e = np.frexp(diff[1:].astype(np.float64))  # highest set bit = e - 1 (exact below 2^53)
+    first = np.empty(n, np.int64)
+    first[0] = 0
+    first[1:] = TREE_DEPTH - (e - 1) // 3
+    count = TREE_DEPTH - first  # inner nodes opened by each leaf, depths first .. 15
+    end = np.cumsum(count)
+    total = int(end[-1])
+    depth = (np.arange(total, dtype=np.int64) - np.repeat(end - count, count) + np.repeat(first, count)).astype(np.int8)
+    owner = np.repeat(code, count)
+    mask = np.zeros(total, np.float64)
+    for a in range(1, TREE_DEPTH):
+        at = np.nonzero(depth == a)[0]
+        above = np.nonzero(depth == a - 1)[0]
+        parent = above[np.searchsorted(above, at) - 1]
+        digit = ((owner[at] >> np.uint64(3 * (TREE_DEPTH - a))) & np.uint64(7)).astype(np.int64)
+        mask += np.bincount(parent, weights=(3 << (2 * digit)).astype(np.float64), minlength=total)
+    del owner
+    leaf_digit = (code & np.uint64(7)).astype(np.int64)
+    mask += np.bincount(end - 1, weights=(2 << (2 * leaf_digit)).astype(np.float64), minlength=total)
+    m = mask.astype(np.int64)
+    body = np.stack([m & 0xFF, m >> 8], 1).astype(np.uint8).tobytes()
+    with open(path, "wb") as f:
+        f.write(_header("# Octomap OcTree binary file", total + n, res))
+        f.write(body)
+    return total + n
+
+
 def write_points(path, xyz, res):
     """Writes the occupied cells of a point set as a ``.bt``/``.ot`` OcTree of
     resolution ``res`` (one occupied depth-16 leaf per distinct cell, key
@@ -245,6 +292,8 @@ def write_points(path, xyz, res):
     key = np.floor(p * (1.0 / res)).astype(np.int64) + (1 << 15)
     if np.any(key < 0) or np.any(key >= (1 << 16)):
         raise ValueError("points outside the OcTree's 2^16-cell range")
+    if str(path).endswith(".bt"):
+        return _write_bt_cells(str(path), key, res)
     packed = np.unique((key[:, 0] << 32) | (key[:, 1] << 16) | key[:, 2])
     idx = np.stack([packed >> 32, (packed >> 16) & 0xFFFF, packed & 0xFFFF], 1)
     return write_octree(path, np.full(len(idx), TREE_DEPTH), idx, np.ones(len(idx), bool), res, check=False)

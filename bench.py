@@ -188,7 +188,8 @@ def main():
             from amcl3d_amd import benchlib
 
             out["extras"] = benchlib.extras(amcl3d_amd, W, plain_value=value,
-                                            building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5), log=log)
+                                            building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5),
+                                            building_map=wl["bm"], log=log)
         except Exception as e:  # extras never hide the main line
             out["extras"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

@@ -359,3 +359,18 @@ def test_planted_mistakes_are_caught(kat, map_bt, monkeypatch):
     bad, _, _, _ = oo.compute_point_cloud(oo.open_octree(map_bt))
     got = (np.asarray(bad, np.float32) - np.float32(BT_MIN)).astype(np.float32)
     assert not np.array_equal(got.view(np.uint32), kat["map_cloud"].view(np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_one_pass_bt_writer_matches_the_general_writer(tmp_path, seed):
+    """write_points' one-pass .bt writer gives the bytes of write_octree for
+    the same depth-16 leaves."""
+    rng = np.random.default_rng(seed)
+    n = [1, 7, 3000, 40000][seed]
+    p = rng.uniform(-40, 40, (n, 3)) * rng.uniform(0.05, 1.0)
+    a = str(tmp_path / "a.bt")
+    b = str(tmp_path / "b.bt")
+    ot.write_points(a, p, 0.05)
+    key = np.unique(np.floor(p / 0.05).astype(np.int64) + (1 << 15), axis=0)
+    ot.write_octree(b, np.full(len(key), 16), key, np.ones(len(key), bool), 0.05)
+    assert open(a, "rb").read() == open(b, "rb").read()
