@@ -48,6 +48,21 @@ class Result(ctypes.Structure):
                 ("deltas", ctypes.c_double * 4), ("superseded", ctypes.c_int)]
 
 
+class NodeParams(ctypes.Structure):
+    _fields_ = [("num_particles", ctypes.c_int), ("resample_interval", ctypes.c_int),
+                ("set_initial_pose", ctypes.c_int)] + \
+               [(k, ctypes.c_double) for k in ("init_x", "init_y", "init_z", "init_a", "init_x_dev", "init_y_dev",
+                                               "init_z_dev", "init_a_dev", "sensor_range", "voxel_size",
+                                               "odom_x_mod", "odom_y_mod", "odom_z_mod", "odom_a_mod",
+                                               "update_rate", "d_th", "a_th", "take_off_height", "alpha")]
+
+
+class NodeOutput(ctypes.Structure):
+    _fields_ = [("published", ctypes.c_int), ("amcl_out", ctypes.c_int), ("has_takenoff", ctypes.c_int),
+                ("base_t", ctypes.c_double * 3), ("base_q", ctypes.c_double * 4),
+                ("odom_t", ctypes.c_double * 3), ("odom_q", ctypes.c_double * 4)]
+
+
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
 _i = ctypes.c_int
@@ -135,6 +150,14 @@ SIGNATURES = {
     "a3d_server_get_particles": [_vp, _i, _vp, _sz],
     "a3d_server_stats": [_vp, _p(_u64), _p(_u64), _p(_u64), _p(_d)],
     "a3d_session_draws": [_u64, _u64, _i, _vp, _vp],
+    "a3d_node_create": [_p(NodeParams), _vp, _u64, _p(_vp)],
+    "a3d_node_destroy": [_vp],
+    "a3d_node_odometry": [_vp, _vp, _vp, _p(NodeOutput)],
+    "a3d_node_range": [_vp, _f, _d, _d, _d],
+    "a3d_node_point_cloud": [_vp, ctypes.c_int64, _vp, _sz, _sz, _p(_i)],
+    "a3d_node_set_initial_pose": [_vp, _vp, _vp, _f, _f, _f, _f],
+    "a3d_node_get_state": [_vp, _vp, _vp, _p(_d), _p(_d), _p(_i), _p(_i), _p(_u64)],
+    "a3d_node_get_filter": [_vp, _p(_vp)],
 }
 _RESTYPES = {"a3d_last_error": ctypes.c_char_p, "a3d_version": ctypes.c_char_p}
 

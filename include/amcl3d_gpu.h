@@ -310,6 +310,60 @@ int a3d_server_drain(a3d_server* s);
 int a3d_server_get_particles(a3d_server* s, int session, float* p7, size_t n);
 /* Counters: requests done, batches launched, evals done, mean batch size. */
 int a3d_server_stats(a3d_server* s, uint64_t* requests, uint64_t* batches, uint64_t* evals, double* busy_ms);
+/* ------------------------------------------------------------------ Node */
+/* The ROS node's glue around the filter (amcl3d/src/Node.cpp), without ROS:
+ * the caller delivers the node's three inputs (odometry transforms, raw
+ * point clouds with the time they arrive, radio ranges) and reads back what
+ * the node publishes.  Parameters as amcl3d/src/Parameters.h:33-73. */
+typedef struct a3d_node a3d_node;
+typedef struct {
+  int num_particles;
+  int resample_interval;
+  int set_initial_pose;
+  double init_x, init_y, init_z, init_a;
+  double init_x_dev, init_y_dev, init_z_dev, init_a_dev;
+  double sensor_range; /* range sigma of update */
+  double voxel_size;
+  double odom_x_mod, odom_y_mod, odom_z_mod, odom_a_mod;
+  double update_rate; /* Hz */
+  double d_th, a_th;
+  double take_off_height;
+  double alpha;
+} a3d_node_params;
+/* What odomCallback publishes (Node.cpp:306-321): the base pose in the world
+ * (lastbase_2_world) and the odom frame in the world (lastodom_2_world), as
+ * translation + quaternion (x, y, z, w); published = 0 when the filter was
+ * not initialised yet and nothing was published. */
+typedef struct {
+  int published;
+  int amcl_out;
+  int has_takenoff;
+  double base_t[3], base_q[4];
+  double odom_t[3], odom_q[4];
+} a3d_node_output;
+/* The node's filter draws from the counter-based generator of served sessions
+ * keyed by `seed`: step 0 for init, then one step per predict (its u01 drives
+ * that update's resample); a3d_session_draws replays them. */
+int a3d_node_create(const a3d_node_params* p, const a3d_grid* grid, uint64_t seed, a3d_node** out);
+int a3d_node_destroy(a3d_node* nd);
+/* Node::odomCallback (Node.cpp:193-324): q = (x, y, z, w). */
+int a3d_node_odometry(a3d_node* nd, const double t[3], const double q[4], a3d_node_output* out);
+/* Node::rangeCallback (Node.cpp:326-345). */
+int a3d_node_range(a3d_node* nd, float range, double ax, double ay, double az);
+/* Node::pointcloudCallback (Node.cpp:112-191) with checkUpdateThresholds
+ * (Node.cpp:347-373): now_ns stands for ros::Time::now(); points as in
+ * a3d_voxel_grid.  *updated = 1 when predict -> update (-> resample) ran. */
+int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_t n, size_t stride, int* updated);
+/* Node::setInitialPose (Node.cpp:375-398). */
+int a3d_node_set_initial_pose(a3d_node* nd, const double t[3], const double q[4], float x_dev, float y_dev,
+                              float z_dev, float a_dev);
+/* mean_p_, the odometry deltas of the last predict, roll_/pitch_, is_odom_,
+ * amcl_out_ and the next draw step. */
+int a3d_node_get_state(const a3d_node* nd, float mean[4], double deltas[4], double* roll, double* pitch, int* is_odom,
+                       int* amcl_out, uint64_t* draw_step);
+/* The node's filter (for its particles). */
+int a3d_node_get_filter(a3d_node* nd, a3d_pf** pf);
+
 /* The draws a session's next `count` steps would use (for tests): z4 for
  * predict (4*N per step) and u01 per step, starting at step `first`. */
 int a3d_session_draws(uint64_t seed, uint64_t step, int num_particles, float* z4, float* u01);
