@@ -10,6 +10,9 @@ spread particle sets make the gathers HBM-bound) is reported beside it.
 map_file: the building map stored as an OcTree .bt file (one occupied leaf
 per occupied cell) and read back the way Grid3d::open reads it.
 
+node_step: the node's point-cloud callback (odometry increment, VoxelGrid,
+predict, update, resample) on the large filter's map, trajectory and clouds.
+
 served: sessions on resident maps, each sending Poisson update requests at
 10 Hz; per session N log-uniform in 1,000..50,000, M uniform in
 2,000..20,000.  p50/p99 latency is request submit -> mean on the host.
@@ -273,7 +276,55 @@ def map_file(amcl3d_amd, bm, sensor_dev=0.05, log=print):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def extras(amcl3d_amd, W, plain_value=None, building=None, building_map=None, log=print):
+def node_step(amcl3d_amd, wl, n=100000, steps=20, warmup=3):
+    """Latency of Node::pointcloudCallback (Node.cpp:112-191) through the
+    library's node: the large filter's building map, trajectory, 20,000-point
+    sensor clouds (voxelised at 0.1 m by the node) and 8 beacons, with the
+    reference launch file's thresholds (amcl3d/launch/amcl3d.launch:
+    voxel_size 0.1, update_rate 100, d_th 0.05, a_th 0.03, resample every
+    update) and n particles.  Odometry is the trajectory relative to its
+    first pose, which is also the initial pose."""
+    import math
+
+    from .node import Node
+
+    traj = wl["traj"]
+    x0, y0, z0, a0 = (float(v) for v in traj[0])
+    params = dict(num_particles=n, resample_interval=0, set_initial_pose=1, init_x=x0, init_y=y0, init_z=z0,
+                  init_a=a0, init_x_dev=0.05, init_y_dev=0.05, init_z_dev=0.05, init_a_dev=0.1, sensor_range=0.53,
+                  voxel_size=0.1, odom_x_mod=0.1, odom_y_mod=0.1, odom_z_mod=0.01, odom_a_mod=0.3,
+                  update_rate=100.0, d_th=0.05, a_th=0.03, take_off_height=-1.0, alpha=0.8)
+    nd = Node(params, wl["grid"], seed=11)
+    c, s_ = math.cos(a0), math.sin(a0)
+    lat = []
+    now = 0
+    for k in range(min(steps + warmup, len(wl["clouds"]))):
+        x, y, z, a = (float(v) for v in traj[k + 1])
+        dx, dy = x - x0, y - y0
+        rel = (c * dx + s_ * dy, -s_ * dx + c * dy, z - z0, a - a0)
+        q = (0.0, 0.0, math.sin(rel[3] / 2), math.cos(rel[3] / 2))
+        if k == 0:
+            nd.odomCallback([0.0, 0.0, 0.0], (0.0, 0.0, 0.0, 1.0))  # initialises the filter
+        nd.odomCallback(rel[:3], q)
+        for r in wl["ranges"][k]:
+            nd.rangeCallback(float(r[0]), float(r[1]), float(r[2]), float(r[3]))
+        now += 100_000_000
+        t0 = time.perf_counter()
+        ran = nd.pointcloudCallback(now, wl["clouds"][k])
+        dt = time.perf_counter() - t0
+        if k >= warmup and ran:
+            lat.append(dt * 1e3)
+    st = nd.state()
+    nd.close()
+    lat = np.sort(np.asarray(lat))
+    return {"workload": "node point-cloud callback: %d particles, 20,000-point clouds voxelised at 0.1 m, 8 beacons, "
+                        "building map; host clouds in, mean on the host" % n,
+            "updates_timed": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "mean_xyz_final": [float(v) for v in st["mean"][:3]]}
+
+
+def extras(amcl3d_amd, W, plain_value=None, building=None, building_map=None, wl=None, log=print):
     bm, grid = room_grid(amcl3d_amd, W)
     out = {"batched": batched(amcl3d_amd, W, bm, grid, map_name="the 200x200x60 room map")}
     log("batched:", out["batched"]["batched_evals_per_s"], "evals/s")
@@ -296,4 +347,7 @@ def extras(amcl3d_amd, W, plain_value=None, building=None, building_map=None, lo
     if building_map is not None:
         out["map_file"] = map_file(amcl3d_amd, building_map, log=log)
         log("map file:", out["map_file"])
+    if wl is not None:
+        out["node_step"] = node_step(amcl3d_amd, wl)
+        log("node step:", out["node_step"])
     return out

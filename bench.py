@@ -189,7 +189,7 @@ def main():
 
             out["extras"] = benchlib.extras(amcl3d_amd, W, plain_value=value,
                                             building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5),
-                                            building_map=wl["bm"], log=log)
+                                            building_map=wl["bm"], wl=wl, log=log)
         except Exception as e:  # extras never hide the main line
             out["extras"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
