@@ -103,13 +103,18 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3, map_name="the
         fl.update_staged(grid, fleet_mod.PLAIN_LOOP)
         fl.synchronize()
 
-    run_loop()
+    # the plain loop at its own best shape, not the fleet's
+    loop_shape, loop_timings = autotune.tune(run_loop, candidates=autotune.FLEET_CANDIDATES, prepare=prepare)
     t_loop = min(_timed(run_loop) for _ in range(reps))
+    autotune.set_shape(shape)
+    prepare(shape)
     ev = fl.evals
     fl.close()
     return {"workload": "%d filters on %s, N log-uniform 500..50k, M 1k..30k, 1..8 beacons" % (nfilters, map_name),
             "evals_per_update": ev, "shape": autotune.shape_name(shape),
             "autotune_s": {autotune.shape_name(k): v for k, v in timings.items()},
+            "plain_loop_shape": autotune.shape_name(loop_shape),
+            "plain_loop_autotune_s": {autotune.shape_name(k): v for k, v in loop_timings.items()},
             "batched_evals_per_s": ev / t_batched, "plain_loop_evals_per_s": ev / t_loop,
             "batched_over_plain_loop": t_loop / t_batched, "batched_weight_kernel_ms": kernel_ms,
             "batched_update_ms": t_batched * 1e3, "plain_loop_update_ms": t_loop * 1e3}

@@ -32,6 +32,13 @@ sys.path.insert(0, ROOT)
 METRIC = "particle·point evals/s; p50/p99 update latency, served vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVAL = 4  # one FP32 prob gather
+# HBM bytes per cloud-weight launch of this workload at its default shape,
+# averaged over the 100 timed steps: rocprofv3 --pmc FETCH_SIZE and --pmc
+# WRITE_SIZE over `python -m amcl3d_amd.profile_run --updates 103 --shape
+# 0,16,1`, each scaled by the counter's reading for a 1 GiB device copy
+# (2.045e9 read + 4.7e6 written).  Counters need their own profiler runs, so
+# the figure is measured apart from this run.
+TRAFFIC_BYTES = 2.05e9
 
 
 def log(*a):
@@ -171,7 +178,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES,
+            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (separate runs, same "
+                            "workload and shape, mean of the 100 timed steps), 0.26x the algorithmic bytes: "
+                            "the L2 serves the gathers of nearby poses",
             "kernel_ms_avg": weight_ms,
             "kernel_launches": launches,
             "algorithmic_bytes_per_launch": evals_per_launch * BYTES_PER_EVAL,
