@@ -173,6 +173,12 @@ struct a3d_node {
   int64_t nextupdate_ns = 0;
   std::vector<float> ranges;  // {r, ax, ay, az} per measurement since the last update
   std::vector<float> z4, vx, vy, vz;
+  // device draws of a predict step: 4 normals per particle, the step's
+  // uniform, and the (seed, step) key
+  hipStream_t stream = nullptr;
+  float* d_z4 = nullptr;
+  float* d_u01 = nullptr;
+  uint64_t* d_key = nullptr;
 };
 
 namespace {
@@ -224,6 +230,12 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
     delete nd;
     return rc;
   }
+  if (hipStreamCreateWithFlags(&nd->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&nd->d_z4, 16 * (size_t)params->num_particles) != hipSuccess ||
+      hipMalloc(&nd->d_u01, 4) != hipSuccess || hipMalloc(&nd->d_key, 16) != hipSuccess) {
+    a3d_node_destroy(nd);
+    return fail(A3D_ERR_NOMEM, "a3d_node_create: cannot allocate the draw buffers");
+  }
   *out = nd;
   return A3D_OK;
 }
@@ -231,6 +243,11 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
 int a3d_node_destroy(a3d_node* nd) {
   if (!nd) return A3D_OK;
   if (nd->pf) a3d_pf_destroy(nd->pf);
+  if (nd->stream) (void)hipStreamSynchronize(nd->stream);
+  (void)hipFree(nd->d_z4);
+  (void)hipFree(nd->d_u01);
+  (void)hipFree(nd->d_key);
+  if (nd->stream) (void)hipStreamDestroy(nd->stream);
   delete nd;
   return A3D_OK;
 }
@@ -336,13 +353,21 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   size_t cur = 0;
   rc = a3d_pf_size(nd->pf, &cur);
   if (rc) return rc;
-  nd->z4.resize(4 * cur);
+  if (cur > (size_t)nd->p.num_particles) return fail(A3D_ERR_STATE, "a3d_node_point_cloud: filter outgrew the node");
+  // this step's draws, as a3d_session_draws gives them, made on the device
+  const uint64_t key[2] = {nd->seed, nd->step};
   float u01 = 0.f;
-  rc = a3d_session_draws(nd->seed, nd->step, (int)cur, nd->z4.data(), &u01);
-  if (rc) return rc;
+  FilterDev f{};
+  f.n = (uint32_t)cur;
+  f.z4 = nd->d_z4;
+  A3D_HIP(hipMemcpyAsync(nd->d_key, key, 16, hipMemcpyHostToDevice, nd->stream));
+  launch_draws(single(f), nd->d_key, nd->d_key + 1, nd->d_u01, nd->stream);
+  A3D_HIP(hipGetLastError());
+  A3D_HIP(hipMemcpyAsync(&u01, nd->d_u01, 4, hipMemcpyDeviceToHost, nd->stream));
+  A3D_HIP(hipStreamSynchronize(nd->stream));
   ++nd->step;
-  rc = a3d_pf_predict(nd->pf, nd->p.odom_x_mod, nd->p.odom_y_mod, nd->p.odom_z_mod, nd->p.odom_a_mod, nd->deltas[0],
-                      nd->deltas[1], nd->deltas[2], nd->deltas[3], nd->z4.data());
+  const double mods[4] = {nd->p.odom_x_mod, nd->p.odom_y_mod, nd->p.odom_z_mod, nd->p.odom_a_mod};
+  rc = a3d_pf_predict_dev(nd->pf, mods, nd->deltas, nd->d_z4);
   if (rc) return rc;
   rc = a3d_pf_update(nd->pf, nd->grid, nd->vx.data(), nd->vy.data(), nd->vz.data(), m, nd->ranges.data(),
                      nd->ranges.size() / 4, nd->p.alpha, nd->p.sensor_range, nd->roll, nd->pitch);
