@@ -213,16 +213,40 @@ __device__ inline ParityFn shfl_up_fn(const ParityFn& f, int d) {
   return r;
 }
 
-// The composition of the maps of all threads before this one (thread order).
+// DPP lane moves of a map (GFX9 row_shr / row_bcast controls): a lane with
+// no source lane, or in a row the row mask leaves out, takes the identity.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ ParityFn dpp_fn(const ParityFn& f) {
+  return ParityFn{dpp_u32<CTRL, ROW_MASK>(0u, f.d0), dpp_u32<CTRL, ROW_MASK>(0u, f.d1),
+                  dpp_u32<CTRL, ROW_MASK>(0u, f.p0), dpp_u32<CTRL, ROW_MASK>(1u, f.p1)};
+}
+
+// Inclusive composition within each row of 16 lanes (shifts by 1, 2, 4, 8).
+__device__ __forceinline__ ParityFn row_inclusive_fn(ParityFn f) {
+  f = fn_then(dpp_fn<0x111, 0xf>(f), f);
+  f = fn_then(dpp_fn<0x112, 0xf>(f), f);
+  f = fn_then(dpp_fn<0x114, 0xf>(f), f);
+  f = fn_then(dpp_fn<0x118, 0xf>(f), f);
+  return f;
+}
+
+// The composition of the maps of all threads before this one (thread order):
+// rows of 16 lanes by DPP shifts, the four rows of a wavefront by row
+// broadcasts (lane 15 into rows 1 and 3, then lane 31 into rows 2 and 3),
+// then the 16 wavefront totals, composed by wavefront 0 in one row.
 __device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int kW = kChainThreads / 64;
+  static_assert(kW == 16, "the wavefront totals are composed in one row of 16 lanes");
   const ParityFn id{0u, 0u, 0u, 1u};
-  ParityFn inc = f;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const ParityFn o = shfl_up_fn(inc, d);
-    if (lane >= d) inc = fn_then(o, inc);
-  }
+  ParityFn inc = row_inclusive_fn(f);
+  inc = fn_then(dpp_fn<0x142, 0xa>(inc), inc);
+  inc = fn_then(dpp_fn<0x143, 0xc>(inc), inc);
   ParityFn ex = shfl_up_fn(inc, 1);
   if (lane == 0) ex = id;
   if (lane == 63) {
@@ -231,20 +255,35 @@ __device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) 
     L.pp[wave] = inc.p0 | (inc.p1 << 1);
   }
   __syncthreads();
-  constexpr int kW = kChainThreads / 64;
-  uint32_t d0[kW], d1[kW], pp[kW];  // all LDS reads first
-#pragma unroll
-  for (int v = 0; v < kW; ++v) {
-    d0[v] = L.d0[v];
-    d1[v] = L.d1[v];
-    pp[v] = L.pp[v];
+  if (wave == 0) {  // exclusive composition of the wavefront totals, in place
+    ParityFn t = id;
+    if (lane < kW) t = ParityFn{L.d0[lane], L.d1[lane], L.pp[lane] & 1u, L.pp[lane] >> 1};
+    const ParityFn te = dpp_fn<0x111, 0xf>(row_inclusive_fn(t));
+    if (lane < kW) {
+      L.d0[lane] = te.d0;
+      L.d1[lane] = te.d1;
+      L.pp[lane] = te.p0 | (te.p1 << 1);
+    }
   }
   __syncthreads();
-  ParityFn w = id;
-#pragma unroll
-  for (int v = 0; v < kW - 1; ++v)
-    if (v < wave) w = fn_then(w, ParityFn{d0[v], d1[v], pp[v] & 1u, pp[v] >> 1});
+  const ParityFn w{L.d0[wave], L.d1[wave], L.pp[wave] & 1u, L.pp[wave] >> 1};
   return fn_then(w, ex);
+}
+
+// The smallest v of the workgroup when v is `none` or increases with the
+// thread index: the v of the first thread whose v is not `none`.
+__device__ __forceinline__ uint32_t block_first(uint32_t v, uint32_t none, ExactLds& L) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t m = __ballot(v != none);
+  const uint32_t src = m ? (uint32_t)__builtin_ctzll(m) : 0u;
+  const uint32_t fv = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+  if (lane == 0) L.first[wave] = m ? fv : none;
+  __syncthreads();
+  uint32_t r = none;
+#pragma unroll
+  for (int w = 0; w < kChainThreads / 64; ++w) r = min(r, L.first[w]);
+  __syncthreads();
+  return r;
 }
 
 __device__ __forceinline__ uint32_t block_min(uint32_t v, ExactLds& L) {
@@ -423,7 +462,7 @@ __device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__
       par = kn & 1u;
       kv[e] = kn;
     }
-    const uint32_t c = block_min(bad, L);
+    const uint32_t c = block_first(bad, tend, L);
     // the terms before c are exact: s_i = k_i * u
     if (prefix) {
 #pragma unroll
