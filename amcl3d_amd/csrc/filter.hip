@@ -185,6 +185,8 @@ constexpr uint32_t kExactHead = 1024;
 struct ExactLds {
   uint32_t d0[kChainThreads / 64], d1[kChainThreads / 64], pp[kChainThreads / 64];
   uint32_t first[kChainThreads / 64];
+  uint32_t kfirst[kChainThreads / 64];
+  uint32_t klast;
   float s;
   float stage[kExactTile];
 };
@@ -268,22 +270,6 @@ __device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) 
   __syncthreads();
   const ParityFn w{L.d0[wave], L.d1[wave], L.pp[wave] & 1u, L.pp[wave] >> 1};
   return fn_then(w, ex);
-}
-
-// The smallest v of the workgroup when v is `none` or increases with the
-// thread index: the v of the first thread whose v is not `none`.
-__device__ __forceinline__ uint32_t block_first(uint32_t v, uint32_t none, ExactLds& L) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t m = __ballot(v != none);
-  const uint32_t src = m ? (uint32_t)__builtin_ctzll(m) : 0u;
-  const uint32_t fv = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
-  if (lane == 0) L.first[wave] = m ? fv : none;
-  __syncthreads();
-  uint32_t r = none;
-#pragma unroll
-  for (int w = 0; w < kChainThreads / 64; ++w) r = min(r, L.first[w]);
-  __syncthreads();
-  return r;
 }
 
 __device__ __forceinline__ uint32_t block_min(uint32_t v, ExactLds& L) {
@@ -441,6 +427,7 @@ __device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__
     uint32_t k = (uint32_t)k0 + (p0 ? pre.d1 : pre.d0);
     uint32_t par = p0 ? pre.p1 : pre.p0;
     uint32_t bad = tend;
+    uint32_t kbad = 0u;  // k before the first irregular step of this thread
     uint32_t kv[kExactPer];
 #pragma unroll
     for (int e = 0; e < kExactPer; ++e) {
@@ -456,27 +443,49 @@ __device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__
       reg = !hard[e] && (reg || xv[e] == 0.0f);  // s + 0 = s
       if (!reg) {
         bad = j0 + e;
+        kbad = k;
         continue;
       }
       k = kn;
       par = kn & 1u;
       kv[e] = kn;
     }
-    const uint32_t c = block_first(bad, tend, L);
+    // c, the first irregular step of the tile (tend if none), and k before
+    // it: per wavefront the first lane with one, then the first such
+    // wavefront; with none, k after the tile's last term
+    uint32_t c = tend, kc = 0u;
+    {
+      const int lane = tid & 63, wave = tid >> 6;
+      const uint64_t mb = __ballot(bad != tend);
+      const int src = mb ? (int)__builtin_ctzll(mb) : 0;
+      const uint32_t fb = (uint32_t)__builtin_amdgcn_readlane((int)bad, src);
+      const uint32_t fk = (uint32_t)__builtin_amdgcn_readlane((int)kbad, src);
+      if (lane == 0) {
+        L.first[wave] = mb ? fb : tend;
+        L.kfirst[wave] = fk;
+      }
+      if (tid == (tend - 1 - pos) / kExactPer) L.klast = k;
+      __syncthreads();
+      // the first of the 16 wavefronts with an irregular step (lane w reads
+      // wavefront w's entry)
+      const uint32_t fw = L.first[lane & 15];
+      const uint64_t mw = __ballot(fw != tend) & 0xffffull;
+      if (mw) {
+        const int w = (int)__builtin_ctzll(mw);
+        c = (uint32_t)__builtin_amdgcn_readlane((int)fw, w);
+        kc = L.kfirst[w];
+      } else {
+        kc = L.klast;
+      }
+      __syncthreads();
+    }
     // the terms before c are exact: s_i = k_i * u
     if (prefix) {
 #pragma unroll
       for (int e = 0; e < kExactPer; ++e)
         if (j0 + e < c && is_checkpoint(j0 + e, n)) prefix[(j0 + e) / kGroup] = (float)((double)(int)kv[e] * u);
     }
-    if (c == pos) {
-      if (tid == 0) L.s = s;
-    } else if (c - 1 >= j0 && c - 1 < j0 + kExactPer) {
-      L.s = (float)((double)(int)kv[c - 1 - j0] * u);
-    }
-    __syncthreads();
-    const float sp = L.s;
-    __syncthreads();
+    const float sp = (float)((double)(int)kc * u);
     if (c < tend) {
       s = sp + x[c];  // the step that may leave the binade: the reference's float add
       if (prefix && tid == 0 && is_checkpoint(c, n)) prefix[c / kGroup] = s;
