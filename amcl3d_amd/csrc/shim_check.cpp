@@ -3,6 +3,7 @@
 // on inputs read from a file, and writes every result to an output file.
 //
 //   shim_check unopened               reference soft failures without a map
+//   shim_check map <file> <sd> <out>  Grid3d::open(map_path, sensor_dev)
 //   shim_check <input> <output>       one init -> predict -> update -> resample
 //
 // Input (little-endian): u32 npts, m, nb, n; f64 min[3], max[3], res,
@@ -12,6 +13,7 @@
 // mean after update[4], resampled particles[7n], weight of the mean pose[1],
 // isIntoMap(mean)[1].
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -54,12 +56,38 @@ int unopened() {
   return (w == 0.f && !in && !g.isOpen()) ? 0 : 1;
 }
 
+// Grid3d::open(map_path, sensor_dev) as the node calls it (Node.cpp:41):
+// writes the grid's sizes, bounds and prob cells; exit 3 when open returns
+// false, as it does for a missing or empty map.
+int open_map(const char* path, double sd, const char* outp) {
+  amcl3d::Grid3d g;
+  if (!g.open(std::string(path), sd)) {
+    std::printf("{\"open\": 0, \"error\": \"%s\"}\n", amcl3d::lastError().c_str());
+    return 3;
+  }
+  uint32_t size[3];
+  double mn[3], mx[3], res, s;
+  if (a3d_grid_get_info(g.handle(), size, mn, mx, &res, &s) != A3D_OK) return 1;
+  const uint64_t cells = (uint64_t)size[0] * size[1] * size[2];
+  std::vector<float> prob(cells);
+  if (a3d_grid_read_cells(g.handle(), 0, cells, prob.data(), nullptr) != A3D_OK) return 1;
+  FILE* o = std::fopen(outp, "wb");
+  if (!o) return 1;
+  const bool ok = std::fwrite(size, 4, 3, o) == 3 && std::fwrite(mn, 8, 3, o) == 3 && std::fwrite(mx, 8, 3, o) == 3 &&
+                  std::fwrite(prob.data(), 4, cells, o) == cells;
+  std::fclose(o);
+  std::printf("{\"open\": 1, \"cells\": %llu}\n", (unsigned long long)cells);
+  return ok ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc == 2 && std::strcmp(argv[1], "unopened") == 0) return unopened();
+  if (argc == 5 && std::strcmp(argv[1], "map") == 0) return open_map(argv[2], std::atof(argv[3]), argv[4]);
   if (argc != 3) {
-    std::fprintf(stderr, "usage: shim_check unopened | shim_check <input> <output>\n");
+    std::fprintf(stderr, "usage: shim_check unopened | shim_check map <file.bt|.ot> <sensor_dev> <output> | "
+                 "shim_check <input> <output>\n");
     return 2;
   }
   FILE* f = std::fopen(argv[1], "rb");

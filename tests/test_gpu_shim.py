@@ -64,3 +64,36 @@ def test_cpp_layer_full_step(gpu, tmp_path):
     ow = oracle.cloud_weight(og, s.cloud, om[0], om[1], om[2], 0.0, 0.0, om[3])
     assert np.float32(w_mean) == np.float32(ow)
     assert bool(into) == bool(oracle.is_into_map(og, om[0], om[1], om[2]))
+
+
+def test_cpp_layer_opens_a_map_file(gpu, tmp_path):
+    """Grid3d::open(map_path, sensor_dev) through the C++ layer, as the node
+    calls it (Node.cpp:41): the .bt read, computeGrid on the device, the
+    cache written; the cells against the oracle chain.  A missing map makes
+    open return false (Grid3dTest.cpp:29-42)."""
+    from amcl3d_amd import octree as ot
+    from oracle import octree as oo
+
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([rng.uniform([0, 0, 0], [6, 4, 0.1], (3000, 3)),
+                          rng.uniform([0, 0, 0], [0.1, 4, 2.5], (1500, 3)),
+                          rng.uniform([2, 1, 0.5], [3, 2, 1.5], (800, 3))])
+    path = str(tmp_path / "site.bt")
+    ot.write_points(path, pts, 0.05)
+    out = tmp_path / "grid.bin"
+    print("shim_check map", path, flush=True)
+    r = subprocess.run([SHIM, "map", path, "0.05", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(out, "rb").read()
+    size = np.frombuffer(raw[:12], np.uint32)
+    mn, mx = np.frombuffer(raw[12:36], np.float64), np.frombuffer(raw[36:60], np.float64)
+    prob = np.frombuffer(raw[60:], np.float32)
+    opts, omn, omx, ores = oo.compute_point_cloud(oo.open_octree(path))
+    assert list(mn) == omn and list(mx) == omx
+    _, oprob = oracle.compute_grid(np.asarray(opts, F32), omn, omx, ores, 0.05, exp_mode=1)
+    assert tuple(size) == oprob.shape[::-1]
+    assert np.array_equal(prob.view(np.int32), oprob.reshape(-1).view(np.int32))
+    assert os.path.exists(str(tmp_path / "site.grid"))
+    r = subprocess.run([SHIM, "map", str(tmp_path / "missing.bt"), "0.05", str(out)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 3 and "Cannot find file" in r.stdout
