@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kChainThreads) void chain_kernel(Filters fs, int wh
 // restarts come densely (terms of mixed sign can cross binades all the time)
 // or s is not finite, the workgroup finishes the sum serially from LDS,
 // still in the reference's order.
-constexpr int kExactPer = 8;                                      // terms per thread per tile
+constexpr int kExactPer = 10;                                     // terms per thread per tile
 constexpr uint32_t kExactTile = kChainThreads * kExactPer;
 // Restarts that each advanced fewer than kExactDenseRun terms: after
 // kExactDense of them in a row (terms of mixed sign crossing binades all the
