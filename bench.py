@@ -39,6 +39,11 @@ BYTES_PER_EVAL = 4  # one FP32 prob gather
 # (2.045e9 read + 4.7e6 written).  Counters need their own profiler runs, so
 # the figure is measured apart from this run.
 TRAFFIC_BYTES = 2.05e9
+# Vector instructions a wavefront issues per 64 evaluations of this kernel
+# (rocprofv3 --pmc SQ_INSTS_VALU over profile_run, steady state), and the
+# chip's issue rate: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64 instruction.
+VALU_PER_64_EVALS = 34.9
+ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
 def log(*a):
@@ -182,6 +187,12 @@ def main():
             "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (separate runs, same "
                             "workload and shape, mean of the 100 timed steps), 0.26x the algorithmic bytes: "
                             "the L2 serves the gathers of nearby poses",
+            "issue": {"valu_instr_per_64_evals": VALU_PER_64_EVALS,
+                      "achieved_wave_instr_per_s": evals_per_launch / 64 * VALU_PER_64_EVALS / (weight_ms * 1e-3),
+                      "peak_wave_instr_per_s": ISSUE_PEAK,
+                      "frac": evals_per_launch / 64 * VALU_PER_64_EVALS / (weight_ms * 1e-3) / ISSUE_PEAK,
+                      "note": "the kernel is bound by vector instruction issue, not HBM: per evaluation the "
+                              "reference's FP64 offset add, float rounding, bounds and cell arithmetic per axis"},
             "kernel_ms_avg": weight_ms,
             "kernel_launches": launches,
             "algorithmic_bytes_per_launch": evals_per_launch * BYTES_PER_EVAL,
