@@ -72,8 +72,9 @@ def main():
                     help="cloud-weight kernel shape 'MAPPING,PPW,ORDER' (e.g. 0,16,1) or 'auto' to time every shape")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the filter path has no multi-GPU form "
-              "(sharding is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU. Filters and sessions are independent, so several GPUs would each "
+              "run their own share with a replica of the maps; there is no multi-GPU form to time here. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import amcl3d_amd

@@ -19,7 +19,8 @@ links (not vendored, absent here), following that library's own structure:
   when log-odds >= 0 (isNodeOccupied, threshold 0.5); bounds as calcMinMax.
 
 Pins (tests/test_octree.py): the reference's occupied map cloud
-(tests/data/mappointcloud_msg.bin: 179,551 centres in leaf order) and the
+(amcl3d/tests/data/mappointcloud_msg.bin, decoded to
+tests/golden/kat_map_cloud.npz: 179,551 centres in leaf order) and the
 bounds and resolution PointCloudToolsTest.cpp:42-53,166-173 assert, for a
 tree rebuilt from those centres; closed-form centres and bounds of single
 leaves at every depth; the error cases of PointCloudToolsTest.cpp:59-103.
