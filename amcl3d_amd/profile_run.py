@@ -1,4 +1,4 @@
-"""Profiling driver: the cloud-weight kernel of the large-filter workload (or,
+"""Profiling workload: the cloud-weight kernel of the large-filter workload (or,
 with ``--fleet``, the persistent batched launch of a 256-filter fleet), then a
 device copy of known size.
 
