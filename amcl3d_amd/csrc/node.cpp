@@ -172,13 +172,21 @@ struct a3d_node {
   Tf base_2_odom = tf_identity();
   int64_t nextupdate_ns = 0;
   std::vector<float> ranges;  // {r, ax, ay, az} per measurement since the last update
-  std::vector<float> z4, vx, vy, vz;
+  std::vector<float> z4;
   // device draws of a predict step: 4 normals per particle, the step's
   // uniform, and the (seed, step) key
   hipStream_t stream = nullptr;
   float* d_z4 = nullptr;
   float* d_u01 = nullptr;
   uint64_t* d_key = nullptr;
+  // the scan on the device: raw points, voxel centroids (x, y, z rows of
+  // v_cap floats), ranges; grown when a larger input arrives
+  void* d_raw = nullptr;
+  size_t raw_cap = 0;  // bytes
+  float* d_v = nullptr;
+  size_t v_cap = 0;  // points
+  float* d_rng = nullptr;
+  size_t rng_cap = 0;  // floats
 };
 
 namespace {
@@ -247,6 +255,9 @@ int a3d_node_destroy(a3d_node* nd) {
   (void)hipFree(nd->d_z4);
   (void)hipFree(nd->d_u01);
   (void)hipFree(nd->d_key);
+  (void)hipFree(nd->d_raw);
+  (void)hipFree(nd->d_v);
+  (void)hipFree(nd->d_rng);
   if (nd->stream) (void)hipStreamDestroy(nd->stream);
   delete nd;
   return A3D_OK;
@@ -336,15 +347,44 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   if (!nd->is_odom) return A3D_OK;
   if (!update_due(nd, now_ns)) return A3D_OK;
   nd->nextupdate_ns = now_ns + duration_ns(1.0 / nd->p.update_rate);
-  size_t m = 0;
-  nd->vx.resize(n ? n : 1);
-  nd->vy.resize(n ? n : 1);
-  nd->vz.resize(n ? n : 1);
+  // the scan and the ranges to the device, then VoxelGrid there
   int rc = A3D_OK;
+  size_t m = 0;
+  const size_t nr = nd->ranges.size();
+  if (n * stride > nd->raw_cap || n > nd->v_cap || nr > nd->rng_cap) {
+    if (n * stride > nd->raw_cap) {
+      (void)hipFree(nd->d_raw);
+      nd->d_raw = nullptr;
+      nd->raw_cap = 0;
+      if (hipMalloc(&nd->d_raw, n * stride) != hipSuccess) return fail(A3D_ERR_NOMEM, "a3d_node_point_cloud: scan");
+      nd->raw_cap = n * stride;
+    }
+    if (n > nd->v_cap) {
+      (void)hipFree(nd->d_v);
+      nd->d_v = nullptr;
+      nd->v_cap = 0;
+      if (hipMalloc(&nd->d_v, 3 * n * sizeof(float)) != hipSuccess)
+        return fail(A3D_ERR_NOMEM, "a3d_node_point_cloud: voxels");
+      nd->v_cap = n;
+    }
+    if (nr > nd->rng_cap) {
+      (void)hipFree(nd->d_rng);
+      nd->d_rng = nullptr;
+      nd->rng_cap = 0;
+      if (hipMalloc(&nd->d_rng, nr * sizeof(float)) != hipSuccess)
+        return fail(A3D_ERR_NOMEM, "a3d_node_point_cloud: ranges");
+      nd->rng_cap = nr;
+    }
+  }
+  float* const d_vx = nd->d_v;
+  float* const d_vy = nd->d_v ? nd->d_v + nd->v_cap : nullptr;
+  float* const d_vz = nd->d_v ? nd->d_v + 2 * nd->v_cap : nullptr;
   if (n) {
-    rc = a3d_voxel_grid(points, n, stride, (float)nd->p.voxel_size, nd->vx.data(), nd->vy.data(), nd->vz.data(), &m);
+    A3D_HIP(hipMemcpy(nd->d_raw, points, n * stride, hipMemcpyHostToDevice));
+    rc = a3d_voxel_grid_dev(nd->d_raw, n, stride, (float)nd->p.voxel_size, d_vx, d_vy, d_vz, &m);
     if (rc) return rc;
   }
+  if (nr) A3D_HIP(hipMemcpy(nd->d_rng, nd->ranges.data(), nr * sizeof(float), hipMemcpyHostToDevice));
   const Tf inc = tf_mul(tf_inv(nd->lastupdatebase_2_odom), nd->base_2_odom);
   nd->deltas[0] = inc.o[0];
   nd->deltas[1] = inc.o[1];
@@ -369,8 +409,8 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   const double mods[4] = {nd->p.odom_x_mod, nd->p.odom_y_mod, nd->p.odom_z_mod, nd->p.odom_a_mod};
   rc = a3d_pf_predict_dev(nd->pf, mods, nd->deltas, nd->d_z4);
   if (rc) return rc;
-  rc = a3d_pf_update(nd->pf, nd->grid, nd->vx.data(), nd->vy.data(), nd->vz.data(), m, nd->ranges.data(),
-                     nd->ranges.size() / 4, nd->p.alpha, nd->p.sensor_range, nd->roll, nd->pitch);
+  rc = a3d_pf_update_dev(nd->pf, nd->grid, d_vx, d_vy, d_vz, m, nd->d_rng, nr / 4, nd->p.alpha, nd->p.sensor_range,
+                         nd->roll, nd->pitch);
   if (rc) return rc;
   rc = a3d_pf_get_mean(nd->pf, nd->mean);
   if (rc) return rc;
