@@ -554,3 +554,32 @@ def test_graph_capture_replays_match_eager(gpu, room):
         assert np.array_equal(graphed.particles().view(np.int32), eager.particles().view(np.int32)), rep
         assert np.array_equal(graphed.getMean(), eager.getMean()), rep
     graph.close()
+
+
+def test_grid_room_every_cell_against_fp64(room):
+    """computeGrid against an FP64 nearest neighbour (scipy cKDTree) on all
+    2.4 million cells of the room map: the error of the float field, stated
+    and bounded (the field itself equals the oracle's bit for bit)."""
+    from scipy.spatial import cKDTree
+
+    bm = room["map"]
+    prob, dist = room["grid"].read_cells(dist=True)
+    sx, sy, sz = (int(v) for v in room["grid"].size)
+    i = np.arange(sx * sy * sz)
+    ix, iy, iz = i % sx, (i // sx) % sy, i // (sx * sy)
+    q = np.stack([bm.min[0] + ix * bm.res, bm.min[1] + iy * bm.res, bm.min[2] + iz * bm.res], -1)
+    q = q.astype(F32).astype(np.float64)  # the sample point as the reference forms it (a float)
+    d64, _ = cKDTree(room["points"].astype(np.float64)).query(q)
+    e64 = d64 ** 2
+    d2 = dist.astype(np.float64)
+    abs_err = np.abs(d2 - e64)
+    rel_err = abs_err / np.maximum(e64, 1e-12)
+    sd = room["sensor_dev"]
+    p64 = 1.0 / (sd * np.sqrt(2 * np.pi)) * np.exp(-e64 * e64 / (2 * sd * sd))
+    dprob = np.abs(prob.astype(np.float64) - p64)
+    _say("computeGrid vs FP64 over %d cells: dist2 max abs %.3g mean abs %.3g max rel %.3g mean rel %.3g; "
+         "max |dprob| %.3g (peak prob %.3g)" % (i.size, abs_err.max(), abs_err.mean(), rel_err.max(), rel_err.mean(),
+                                                dprob.max(), p64.max()))
+    assert i.size >= 1_000_000
+    assert rel_err.max() < 1e-5
+    assert dprob.max() < 1e-5 * p64.max()
