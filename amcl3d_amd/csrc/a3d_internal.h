@@ -181,8 +181,12 @@ int build_grid_device(const float* d_xyz, size_t n, const GridDev& g, float* d_p
                       double sensor_dev, hipStream_t s);
 
 // pcl::VoxelGrid as Node.cpp:130-141 applies it; synchronous on stream s.
+// ws (optional): device buffers kept between calls (voxel_workspace_new).
+struct VoxelWorkspace;
+VoxelWorkspace* voxel_workspace_new();
+void voxel_workspace_free(VoxelWorkspace* w);
 int voxel_grid_device(const void* d_pts, size_t n, size_t stride, float leaf, float* d_x, float* d_y, float* d_z,
-                      size_t* out_n, hipStream_t s);
+                      size_t* out_n, hipStream_t s, VoxelWorkspace* ws = nullptr);
 
 // Counter-based draws for served sessions (Philox4x32-10 + Box-Muller).
 // Writes 4 normals per particle to each filter's z4 and one uniform per

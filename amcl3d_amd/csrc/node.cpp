@@ -187,6 +187,7 @@ struct a3d_node {
   size_t v_cap = 0;  // points
   float* d_rng = nullptr;
   size_t rng_cap = 0;  // floats
+  VoxelWorkspace* vws = nullptr;  // VoxelGrid's sort buffers
 };
 
 namespace {
@@ -240,7 +241,8 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
   }
   if (hipStreamCreateWithFlags(&nd->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&nd->d_z4, 16 * (size_t)params->num_particles) != hipSuccess ||
-      hipMalloc(&nd->d_u01, 4) != hipSuccess || hipMalloc(&nd->d_key, 16) != hipSuccess) {
+      hipMalloc(&nd->d_u01, 4) != hipSuccess || hipMalloc(&nd->d_key, 16) != hipSuccess ||
+      !(nd->vws = voxel_workspace_new())) {
     a3d_node_destroy(nd);
     return fail(A3D_ERR_NOMEM, "a3d_node_create: cannot allocate the draw buffers");
   }
@@ -258,6 +260,7 @@ int a3d_node_destroy(a3d_node* nd) {
   (void)hipFree(nd->d_raw);
   (void)hipFree(nd->d_v);
   (void)hipFree(nd->d_rng);
+  voxel_workspace_free(nd->vws);
   if (nd->stream) (void)hipStreamDestroy(nd->stream);
   delete nd;
   return A3D_OK;
@@ -343,6 +346,8 @@ int a3d_node_range(a3d_node* nd, float range, double ax, double ay, double az) {
 int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_t n, size_t stride, int* updated) {
   // Node::pointcloudCallback (Node.cpp:112-191)
   if (!nd || (n && !points) || !updated) return fail(A3D_ERR_ARG, "a3d_node_point_cloud: null argument");
+  if (n && (stride < 12 || stride % 4 != 0 || n > 0x7fffffffull))
+    return fail(A3D_ERR_ARG, "a3d_node_point_cloud: stride must be >= 12 and a multiple of 4, n < 2^31");
   *updated = 0;
   if (!nd->is_odom) return A3D_OK;
   if (!update_due(nd, now_ns)) return A3D_OK;
@@ -381,7 +386,7 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   float* const d_vz = nd->d_v ? nd->d_v + 2 * nd->v_cap : nullptr;
   if (n) {
     A3D_HIP(hipMemcpy(nd->d_raw, points, n * stride, hipMemcpyHostToDevice));
-    rc = a3d_voxel_grid_dev(nd->d_raw, n, stride, (float)nd->p.voxel_size, d_vx, d_vy, d_vz, &m);
+    rc = voxel_grid_device(nd->d_raw, n, stride, (float)nd->p.voxel_size, d_vx, d_vy, d_vz, &m, nullptr, nd->vws);
     if (rc) return rc;
   }
   if (nr) A3D_HIP(hipMemcpy(nd->d_rng, nd->ranges.data(), nr * sizeof(float), hipMemcpyHostToDevice));

@@ -165,35 +165,54 @@ __global__ void voxel_copy_kernel(const uint8_t* pts, size_t stride, uint32_t n,
 
 }  // namespace
 
+// Device buffers of voxel_grid_device, kept by a caller that filters many
+// clouds: the meta record, six arrays of `cap` words (keys, values, their
+// sorted copies, voxel heads and positions) and the sort / scan temporary.
+struct VoxelWorkspace {
+  VoxelMeta* meta = nullptr;
+  uint32_t* buf = nullptr;
+  size_t cap = 0;
+  void* tmp = nullptr;
+  size_t tmp_cap = 0;
+};
+
+VoxelWorkspace* voxel_workspace_new() { return new VoxelWorkspace(); }
+
+void voxel_workspace_free(VoxelWorkspace* w) {
+  if (!w) return;
+  (void)hipFree(w->meta);
+  (void)hipFree(w->buf);
+  (void)hipFree(w->tmp);
+  delete w;
+}
+
 // d_pts: n points, `stride` bytes apart (x, y, z the first three floats).
 // Writes the filtered cloud as d_x, d_y, d_z (capacity n) and its size.
+// ws: buffers to reuse (grown as needed), or null for buffers of this call.
 int voxel_grid_device(const void* d_pts, size_t n, size_t stride, float leaf, float* d_x, float* d_y, float* d_z,
-                      size_t* out_n, hipStream_t s) {
+                      size_t* out_n, hipStream_t s, VoxelWorkspace* ws) {
   *out_n = 0;
   if (n == 0) return A3D_OK;
   const uint8_t* pts = static_cast<const uint8_t*>(d_pts);
   const uint32_t nn = (uint32_t)n;
   const uint32_t blocks = (nn + 255) / 256;
-  VoxelMeta* meta = nullptr;
+  VoxelWorkspace local;
+  VoxelWorkspace& w = ws ? *ws : local;
   uint32_t *key = nullptr, *val = nullptr, *key2 = nullptr, *val2 = nullptr, *head = nullptr, *pos = nullptr;
-  void* tmp = nullptr;
   size_t tmp_bytes = 0, tmp2 = 0;
   auto cleanup = [&]() {
-    (void)hipFree(meta);
-    (void)hipFree(key);
-    (void)hipFree(val);
-    (void)hipFree(key2);
-    (void)hipFree(val2);
-    (void)hipFree(head);
-    (void)hipFree(pos);
-    (void)hipFree(tmp);
+    if (ws) return;
+    (void)hipFree(local.meta);
+    (void)hipFree(local.buf);
+    (void)hipFree(local.tmp);
   };
 #define A3D_TRY(call)           \
   if (!hip_ok((call), #call)) { \
     cleanup();                  \
     return A3D_ERR_HIP;         \
   }
-  A3D_TRY(hipMalloc(&meta, sizeof(VoxelMeta)));
+  if (!w.meta) A3D_TRY(hipMalloc(&w.meta, sizeof(VoxelMeta)));
+  VoxelMeta* const meta = w.meta;
   voxel_meta_kernel<<<1, kVT, 0, s>>>(pts, nn, stride, leaf, meta);
   A3D_TRY(hipGetLastError());
   VoxelMeta hm{};
@@ -211,18 +230,32 @@ int voxel_grid_device(const void* d_pts, size_t n, size_t stride, float leaf, fl
     *out_n = n;
     return A3D_OK;
   }
-  A3D_TRY(hipMalloc(&key, n * 4));
-  A3D_TRY(hipMalloc(&val, n * 4));
-  A3D_TRY(hipMalloc(&key2, n * 4));
-  A3D_TRY(hipMalloc(&val2, n * 4));
-  A3D_TRY(hipMalloc(&head, n * 4));
-  A3D_TRY(hipMalloc(&pos, n * 4));
+  if (w.cap < n) {
+    (void)hipFree(w.buf);
+    w.buf = nullptr;
+    w.cap = 0;
+    A3D_TRY(hipMalloc(&w.buf, 6 * n * 4));
+    w.cap = n;
+  }
+  key = w.buf;
+  val = key + w.cap;
+  key2 = val + w.cap;
+  val2 = key2 + w.cap;
+  head = val2 + w.cap;
+  pos = head + w.cap;
   voxel_key_kernel<<<blocks, 256, 0, s>>>(pts, nn, stride, meta, key, val);
   A3D_TRY(hipGetLastError());
   A3D_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key2, val, val2, (int)nn, 0, 32, s));
   A3D_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, head, pos, (int)nn, s));
   tmp_bytes = std::max(tmp_bytes, tmp2);
-  A3D_TRY(hipMalloc(&tmp, tmp_bytes));
+  if (w.tmp_cap < tmp_bytes) {
+    (void)hipFree(w.tmp);
+    w.tmp = nullptr;
+    w.tmp_cap = 0;
+    A3D_TRY(hipMalloc(&w.tmp, tmp_bytes));
+    w.tmp_cap = tmp_bytes;
+  }
+  void* const tmp = w.tmp;
   A3D_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key2, val, val2, (int)nn, 0, 32, s));
   voxel_head_kernel<<<blocks, 256, 0, s>>>(key2, nn, head);
   A3D_TRY(hipGetLastError());
