@@ -216,20 +216,48 @@ __host__ __device__ inline size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 
 __host__ __device__ inline size_t sc_sums(uint32_t n) { return sc_tot(n) + 2 * ((size_t)n / 1024 + 2); }
 __host__ __device__ inline size_t scratch_floats(uint32_t n) { return sc_sums(n) + 32; }
 
-// Event pairs around the cloud-weight launches, resolved on demand.
+// Event pairs around the cloud-weight launches.  Completed pairs are folded
+// into acc_ms and reused as later launches take pairs, so a caller that never
+// reads the timing (a node's callback loop) holds at most kMaxPairs pairs;
+// while that many are still in flight, a launch goes untimed.
 struct TimingPool {
+  static constexpr size_t kMaxPairs = 64;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
   double acc_ms = 0.0;
   uint64_t launches = 0;
   uint64_t evals = 0;
 
+  // Moves the pairs whose end event has completed from pending to spare.  A
+  // query of an event still in flight answers hipErrorNotReady, which the
+  // runtime may keep as the thread's last error; that is cleared again unless
+  // an error was already pending before the queries.
+  void reap() {
+    const bool clean = hipPeekAtLastError() == hipSuccess;
+    size_t k = 0;
+    for (size_t i = 0; i < pending.size(); ++i) {
+      const auto e = pending[i];
+      if (hipEventQuery(e.second) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) acc_ms += ms;
+        spare.push_back(e);
+      } else {
+        pending[k++] = e;
+      }
+    }
+    pending.resize(k);
+    if (clean) (void)hipGetLastError();
+  }
+  size_t held() const { return pending.size() + spare.size(); }
+
   std::pair<hipEvent_t, hipEvent_t> take() {
+    if (spare.empty()) reap();
     if (!spare.empty()) {
       auto e = spare.back();
       spare.pop_back();
       return e;
     }
+    if (held() >= kMaxPairs) return {nullptr, nullptr};
     std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
     if (hipEventCreate(&e.first) != hipSuccess) return {nullptr, nullptr};
     if (hipEventCreate(&e.second) != hipSuccess) {

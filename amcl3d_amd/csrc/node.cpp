@@ -230,6 +230,11 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
   *out = nullptr;
   if (params->num_particles <= 0 || !(params->update_rate > 0) || !(params->voxel_size > 0))
     return fail(A3D_ERR_ARG, "a3d_node_create: num_particles, update_rate and voxel_size must be positive");
+  // VoxelGrid takes the leaf as a float (setLeafSize(float, float, float),
+  // Node.cpp:134): it must stay positive and finite after that rounding
+  const float leaf = (float)params->voxel_size;
+  if (!(leaf > 0.0f) || !std::isfinite(leaf))
+    return fail(A3D_ERR_ARG, "a3d_node_create: voxel_size must be a positive, finite float");
   a3d_node* nd = new a3d_node();
   nd->p = *params;
   nd->grid = grid;

@@ -181,6 +181,11 @@ struct a3d_server {
   void run();
   bool launch(Slot& s);
   void retire(Slot& s);
+  // Waits for every batch enqueued on the slot streams.
+  void sync_slots() {
+    for (auto& s : slots)
+      if (s.stream) (void)hipStreamSynchronize(s.stream);
+  }
 };
 
 static int alloc_session(a3d_server* S, Session& se) {
@@ -287,7 +292,10 @@ bool a3d_server::launch(Slot& s) {
   for (auto& r : s.reqs) mixed |= sessions[r.session].map != map0;
   if (mixed)
     for (size_t i = 0; i < nf; ++i) s.h_filters[i].gdev = d_maps[sessions[s.reqs[i].session].map];
-  if (!grow_dev(s.d_items, s.items_cap, s.h_items.size())) return false;
+  if (!grow_dev(s.d_items, s.items_cap, s.h_items.size())) {
+    worker_error = "serving: staging allocation failed";
+    return false;
+  }
   hipStream_t st = s.stream;
   bool ok = hipMemcpyAsync(s.d_data, s.h_data, off * 4, hipMemcpyHostToDevice, st) == hipSuccess &&
             hipMemcpyAsync(s.d_filters, s.h_filters.data(), nf * sizeof(FilterDev), hipMemcpyHostToDevice, st) ==
@@ -394,6 +402,7 @@ void a3d_server::run() {
       }
       if (s.reqs.empty()) continue;
       if (!launch(s)) {
+        if (worker_error.empty()) worker_error = "serving: launch failed";
         stop = true;
         break;
       }
@@ -585,6 +594,9 @@ int a3d_server_close_session(a3d_server* S, int session) {
   for (auto& r : S->pending)
     if (r.session == session) return fail(A3D_ERR_BUSY, "a3d_server_close_session: requests pending");
   S->cv.wait(lk, [&]() { return !S->sessions[session].busy || S->stop; });
+  // still busy: the worker stopped on an error with the session's batch
+  // enqueued; its stream may still read or write the session's state
+  if (S->sessions[session].busy) S->sync_slots();
   free_session(S, S->sessions[session]);
   return A3D_OK;
 }
@@ -674,6 +686,12 @@ int a3d_server_get_particles(a3d_server* S, int session, float* p7, size_t n) {
   if (session < 0 || session >= (int)S->sessions.size() || !S->sessions[session].open)
     return fail(A3D_ERR_ARG, "a3d_server_get_particles: bad session");
   S->cv.wait(lk, [&]() { return !S->sessions[session].busy || S->stop; });
+  if (S->sessions[session].busy) {
+    // the worker stopped with this session's batch enqueued: its state is
+    // not the result of a completed update
+    S->sync_slots();
+    return fail(A3D_ERR_HIP, S->worker_error.empty() ? "a3d_server_get_particles: server stopped" : S->worker_error);
+  }
   Session& se = S->sessions[session];
   if (n != se.n) return fail(A3D_ERR_ARG, "a3d_server_get_particles: n must equal the session's particle count");
   std::vector<float> soa(7 * n);

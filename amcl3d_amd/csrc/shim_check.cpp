@@ -4,7 +4,9 @@
 //
 //   shim_check unopened               reference soft failures without a map
 //   shim_check map <file> <sd> <out>  Grid3d::open(map_path, sensor_dev)
-//   shim_check <input> <output>       one init -> predict -> update -> resample
+//   shim_check <input> <output> [neg] one init -> predict -> update -> resample;
+//                                     with neg, init is given -n particles
+//                                     (the reference resizes to abs(n))
 //
 // Input (little-endian): u32 npts, m, nb, n; f64 min[3], max[3], res,
 // sensor_dev, mods[4], deltas[4], alpha, sigma; f32 u01, pose[4], devs[4];
@@ -85,9 +87,10 @@ int open_map(const char* path, double sd, const char* outp) {
 int main(int argc, char** argv) {
   if (argc == 2 && std::strcmp(argv[1], "unopened") == 0) return unopened();
   if (argc == 5 && std::strcmp(argv[1], "map") == 0) return open_map(argv[2], std::atof(argv[3]), argv[4]);
-  if (argc != 3) {
+  const bool neg = argc == 4 && std::strcmp(argv[3], "neg") == 0;
+  if (argc != 3 && !neg) {
     std::fprintf(stderr, "usage: shim_check unopened | shim_check map <file.bt|.ot> <sensor_dev> <output> | "
-                 "shim_check <input> <output>\n");
+                 "shim_check <input> <output> [neg]\n");
     return 2;
   }
   FILE* f = std::fopen(argv[1], "rb");
@@ -129,7 +132,7 @@ int main(int argc, char** argv) {
     const amcl3d::Particle q = pf.getMean();
     out.insert(out.end(), {q.x, q.y, q.z, q.a});
   };
-  pf.init((int)n, pose[0], pose[1], pose[2], pose[3], devs[0], devs[1], devs[2], devs[3]);
+  pf.init(neg ? -(int)n : (int)n, pose[0], pose[1], pose[2], pose[3], devs[0], devs[1], devs[2], devs[3]);
   if (!pf.isInitialized()) {
     std::fprintf(stderr, "shim_check: init failed: %s\n", amcl3d::lastError().c_str());
     return 1;

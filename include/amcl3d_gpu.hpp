@@ -23,8 +23,9 @@
  *    std::uniform_real_distribution<float>(0, 1), the reference's generator
  *    family, ParticleFilter.cpp:22,246-256).  The device applies each
  *    particle's deviation to the standard normal.
- *  - Grid3d::open takes the map's occupied points and bounds (the octomap
- *    reader is not part of this library) or a .grid file plus bounds.
+ *  - Grid3d::open(map_path, sensor_dev) reads the OcTree file as the
+ *    reference does; two more overloads take the map's occupied points and
+ *    bounds, or a .grid file plus bounds.
  *  - Failures of the device path (no GPU, out of memory) are reported by
  *    lastError(); the reference has no such failure mode.
  */
@@ -33,6 +34,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <functional>
 #include <memory>
 #include <random>
@@ -184,8 +186,11 @@ class ParticleFilter {
   // ParticleFilter::init (ParticleFilter.cpp:46-95).
   void init(int num_particles, float x_init, float y_init, float z_init, float a_init, float x_dev, float y_dev,
             float z_dev, float a_dev) {
-    if (!pf_ || num_particles <= 0) return;
-    z4_.resize(4 * (size_t)num_particles);
+    // p_.resize(abs(num_particles)) (ParticleFilter.cpp:51): a negative
+    // count is taken by its magnitude
+    const size_t n = (size_t)std::llabs((long long)num_particles);
+    if (!pf_ || n == 0) return;
+    z4_.resize(4 * n);
     draws_->normals(z4_.data(), z4_.size());
     a3d_pf_init(pf_, num_particles, x_init, y_init, z_init, a_init, x_dev, y_dev, z_dev, a_dev, z4_.data());
   }

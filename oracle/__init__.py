@@ -63,6 +63,9 @@ def lib():
         L.orc_cloud_weight_batch.argtypes = [_f32p, _u32p, _f64p, _f64p, ctypes.c_double, _f32p, _f32p, _f32p,
                                              _sz, _f32p, _f32p, _f32p, _f32p, _sz, ctypes.c_float,
                                              ctypes.c_float, _f32p, ctypes.c_int]
+        L.orc_cloud_cells.argtypes = [_u32p, _f64p, _f64p, ctypes.c_double, _f32p, _f32p, _f32p, _sz, _f32p, _f32p,
+                                      _f32p, _f32p, _sz, ctypes.c_float, ctypes.c_float,
+                                      np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]
         L.orc_is_into_map.argtypes = [_u32p, _f64p, _f64p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float]
         L.orc_is_into_map.restype = ctypes.c_int
@@ -151,6 +154,19 @@ def cloud_weight_batch(g, cloud, tx, ty, tz, yaw, roll=0.0, pitch=0.0, nthreads=
     out = np.empty(n, np.float32)
     lib().orc_cloud_weight_batch(g.prob, g.size, g.min, g.max, g.res, x, y, z, c.shape[0], _f32(tx), _f32(ty),
                                  _f32(tz), _f32(yaw), n, roll, pitch, out, _nthreads(nthreads))
+    return out
+
+
+def cloud_cells(g, cloud, tx, ty, tz, yaw, roll=0.0, pitch=0.0):
+    """The cells computeCloudWeight gathers: (n, m) int64, the cell index of
+    point k under pose i, or -1 for a point outside the map.  Needs only the
+    grid's geometry (g.prob may be None)."""
+    c = _f32(cloud).reshape(-1, 3)
+    x, y, z = (np.ascontiguousarray(c[:, k]) for k in range(3))
+    n = len(tx)
+    out = np.empty((n, c.shape[0]), np.int64)
+    lib().orc_cloud_cells(g.size, g.min, g.max, g.res, x, y, z, c.shape[0], _f32(tx), _f32(ty), _f32(tz), _f32(yaw),
+                          n, roll, pitch, out.reshape(-1))
     return out
 
 

@@ -70,10 +70,12 @@ void rotation(float roll, float pitch, float yaw, float r[9]) {
   r[8] = (float)(cp * cr);
 }
 
-// Grid3d.cpp:133-199
-float cloud_weight(const GridView& g, const float* px, const float* py, const float* pz, size_t m, float tx,
-                   float ty, float tz, float roll, float pitch, float yaw) {
-  if (!g.prob) return 0;
+// The cell each point of the cloud lands in under one pose, as
+// Grid3d.cpp:139-196 computes it: visit(k, idx) for every point k inside the
+// map, in cloud order.  Shared by cloud_weight and orc_cloud_cells.
+template <class Visit>
+void cloud_cells(const GridView& g, const float* px, const float* py, const float* pz, size_t m, float tx, float ty,
+                 float tz, float roll, float pitch, float yaw, Visit visit) {
   float r[9];
   rotation(roll, pitch, yaw, r);
   const double sx = g.max[0] - g.min[0];
@@ -86,8 +88,6 @@ float cloud_weight(const GridView& g, const float* px, const float* py, const fl
   const uint64_t step_z = (uint64_t)g.size[0] * g.size[1];
   const uint64_t grid_size = step_z * g.size[2];
   const float error_z = 0;
-  float weight = 0.f;
-  int n = 0;
   for (size_t k = 0; k < m; ++k) {
     const float x = px[k], y = py[k], z = pz[k] + error_z;
     const float nx = (float)((double)(x * r[0] + y * r[1] + z * r[2]) + ox);
@@ -99,13 +99,23 @@ float cloud_weight(const GridView& g, const float* px, const float* py, const fl
       const uint32_t iz = (uint32_t)std::floor(nz / g.res);
       if (ix < g.size[0] && iy < g.size[1] && iz < g.size[2]) {
         const uint64_t idx = ix + iy * step_y + iz * step_z;
-        if (idx < grid_size) {
-          weight += g.prob[idx];
-          n += 1;
-        }
+        if (idx < grid_size) visit(k, idx);
       }
     }
   }
+}
+
+// Grid3d.cpp:133-199: the float sum of prob over the points inside the map,
+// in cloud order, divided by their count when there are more than 10.
+float cloud_weight(const GridView& g, const float* px, const float* py, const float* pz, size_t m, float tx,
+                   float ty, float tz, float roll, float pitch, float yaw) {
+  if (!g.prob) return 0;
+  float weight = 0.f;
+  int n = 0;
+  cloud_cells(g, px, py, pz, m, tx, ty, tz, roll, pitch, yaw, [&](size_t, uint64_t idx) {
+    weight += g.prob[idx];
+    n += 1;
+  });
   return (n <= 10) ? 0 : weight / n;
 }
 
@@ -351,6 +361,22 @@ void orc_cloud_weight_batch(const float* prob, const uint32_t size[3], const dou
   for (int t = 1; t < nthreads; ++t) th.emplace_back(work, t);
   work(0);
   for (auto& x : th) x.join();
+}
+
+// Cells gathered by computeCloudWeight(cloud, t_i, roll, pitch, yaw_i):
+// out[i*m + k] = the cell index of point k under pose i, or -1 when the point
+// is outside the map.
+void orc_cloud_cells(const uint32_t size[3], const double mn[3], const double mx[3], double res, const float* px,
+                     const float* py, const float* pz, size_t m, const float* tx, const float* ty, const float* tz,
+                     const float* yaw, size_t n, float roll, float pitch, int64_t* out) {
+  static const float dummy = 0.f;  // the map is open
+  GridView g{&dummy, {size[0], size[1], size[2]}, {mn[0], mn[1], mn[2]}, {mx[0], mx[1], mx[2]}, res};
+  for (size_t i = 0; i < n; ++i) {
+    int64_t* o = out + i * m;
+    for (size_t k = 0; k < m; ++k) o[k] = -1;
+    cloud_cells(g, px, py, pz, m, tx[i], ty[i], tz[i], roll, pitch, yaw[i],
+                [&](size_t k, uint64_t idx) { o[k] = (int64_t)idx; });
+  }
 }
 
 int orc_is_into_map(const uint32_t size[3], const double mn[3], const double mx[3], int open, float x, float y,

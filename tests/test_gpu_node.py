@@ -22,6 +22,7 @@ PARAMS = dict(num_particles=600, resample_interval=1, set_initial_pose=1, init_x
               voxel_size=0.1, odom_x_mod=0.1, odom_y_mod=0.1, odom_z_mod=0.1, odom_a_mod=0.3, update_rate=10.0,
               d_th=0.1, a_th=0.05, take_off_height=0.5, alpha=0.8)
 SEED = 77
+SCAN_SIZES = [1500, 1500, 3000, 700, 700, 5200, 1200, 4000, 300]
 
 
 def _say(*a):
@@ -86,10 +87,16 @@ def test_node_matches_the_oracle_on_a_scripted_flight(gpu):
             r = math.dist(w[:3], an) + rng.normal(0, 0.05)
             nd.rangeCallback(r, *an)
             ref.range(r, *an)
-        cloud = W.sensor_cloud(bm, (w[0], w[1], max(w[2], 0.3), w[3]), 1500, 0.02, rng)
+        # scan sizes grow, shrink and grow again (the node's device buffers
+        # and VoxelGrid workspace regrow, then run with capacity above the
+        # scan size); every other scan arrives as pcl::PointXYZ (16 bytes)
+        m = SCAN_SIZES[k % len(SCAN_SIZES)]
+        cloud = W.sensor_cloud(bm, (w[0], w[1], max(w[2], 0.3), w[3]), m, 0.02, rng)
+        if k % 2:
+            cloud = np.concatenate([cloud, np.ones((m, 1), np.float32)], 1)
         now += 60_000_000 if k % 3 else 30_000_000
         ua = nd.pointcloudCallback(now, cloud)
-        ub = ref.point_cloud(now, cloud)
+        ub = ref.point_cloud(now, cloud[:, :3])
         assert ua == ub, k
         updates += ua
         st = nd.state()
@@ -101,3 +108,33 @@ def test_node_matches_the_oracle_on_a_scripted_flight(gpu):
     assert updates >= 10
     assert ref.amcl_out  # the jump was seen
     assert np.array_equal(nd.particles(), ref.particles)
+
+
+def test_node_callbacks_keep_timing_bounded(gpu):
+    """Hundreds of point-cloud callbacks that never read the filter's kernel
+    timing: the timing events are recycled (the pool holds at most 64 pairs),
+    and every update is still timed and counted."""
+    bm = W.room_map(2)
+    g = gpu.Grid3d()
+    g.open(bm.points(), bm.min, bm.max, bm.res, 0.1)
+    from amcl3d_amd.node import Node
+
+    params = dict(PARAMS, num_particles=200, resample_interval=0)
+    nd = Node(params, g, seed=5)
+    rng = np.random.default_rng(8)
+    q = on.quat_rpy(0.0, 0.0, 0.4)
+    nd.odomCallback([0.0, 0.0, 1.0], q)
+    cloud = W.sensor_cloud(bm, (6.0, 7.0, 1.3, 0.4), 800, 0.02, rng)  # the same scan each time
+    now, updates = 0, 0
+    _say("300 node callbacks")
+    for k in range(300):
+        nd.odomCallback([0.15 * (k + 1), 0.0, 1.0], q)
+        now += 120_000_000
+        updates += nd.pointcloudCallback(now, cloud)
+    assert updates >= 250
+    pf = ctypes.c_void_p()
+    _lib.call("a3d_node_get_filter", nd._h, ctypes.byref(pf))
+    ms, launches, evals = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+    _lib.call("a3d_pf_timing", pf.value, 1, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(evals))
+    _say("updates", updates, "timed launches", launches.value, "avg ms", ms.value)
+    assert launches.value == updates and ms.value > 0

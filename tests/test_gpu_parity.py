@@ -317,6 +317,10 @@ def test_init_bit_exact(gpu):
         op, om = oracle.pf_init(n, (2.87080164308, 1.63820032984, 0.21460881682, 0.144), (0.05, 0.05, 0.05, 0.1), z)
         assert np.array_equal(pf.particles().view(np.int32), op.view(np.int32)), n
         assert np.array_equal(pf.getMean().view(np.int32), om.view(np.int32)), n
+        # p_.resize(abs(num_particles)) (ParticleFilter.cpp:51): -n gives the same filter
+        pf2 = gpu.ParticleFilter()
+        pf2.init(-n, 2.87080164308, 1.63820032984, 0.21460881682, 0.144, 0.05, 0.05, 0.05, 0.1, z)
+        assert np.array_equal(pf2.particles().view(np.int32), op.view(np.int32)), -n
 
 
 def test_predict_keeps_mean_and_is_bit_exact(gpu):

@@ -14,7 +14,11 @@ F32 = np.float32
 SHIM = os.path.join(os.path.dirname(_lib.LIB_PATH), "shim_check")
 
 
-def test_cpp_layer_full_step(gpu, tmp_path):
+@pytest.mark.parametrize("neg", [False, True], ids=["n", "minus-n"])
+def test_cpp_layer_full_step(gpu, tmp_path, neg):
+    """init -> predict -> update -> resample through the C++ layer; with
+    minus-n, init is given -n particles, which the reference resizes to
+    abs(n) (ParticleFilter.cpp:51), so the results are the same."""
     c = W.room_case(4, n=700, m=1500)
     bm, s = c["map"], c["step"]
     pts = bm.points().astype(F32)
@@ -32,8 +36,9 @@ def test_cpp_layer_full_step(gpu, tmp_path):
         for a in (pts, s.cloud, s.ranges, z_init, s.z4):
             np.ascontiguousarray(a, F32).tofile(f)
     out = tmp_path / "out.bin"
-    print("shim_check: C++ layer, room seed 4, n=700 m=1500", flush=True)
-    r = subprocess.run([SHIM, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print("shim_check: C++ layer, room seed 4, n=700 m=1500, negative count", neg, flush=True)
+    r = subprocess.run([SHIM, str(inp), str(out)] + (["neg"] if neg else []), capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, F32)
     k = 0

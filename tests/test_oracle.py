@@ -174,6 +174,30 @@ def test_cloud_weight_cells(roll, pitch, yaw):
     exp, n = _expected_weight(g, cloud, t, R)
     assert n == 500
     assert abs(w - exp) <= 1e-6 * exp
+    # the gathered cells themselves are the analytic ones
+    cl = oracle.cloud_cells(g, cloud, [t[0]], [t[1]], [t[2]], [yaw], roll, pitch)[0]
+    assert np.array_equal(cl, cells[:, 0] + g.size[0] * (cells[:, 1] + g.size[1] * cells[:, 2]))
+
+
+def test_cloud_cells_sum_to_the_weight():
+    """cloud_cells lists the cells cloud_weight adds: their prob summed in
+    cloud order as floats, over more than 10 points, divided by the count, is
+    the weight; points outside the map (here a third of them) are -1."""
+    c = W.room_case(2, n=64, m=900)
+    bm, s, p = c["map"], c["step"], c["particles"]
+    rng = np.random.default_rng(4)
+    g = oracle.GridMeta(bm.min, bm.max, bm.res, rng.uniform(0, 4, int(np.prod(bm.size))).astype(F32))
+    cloud = s.cloud.copy()
+    cloud[::3] += F32(40.0)
+    w = oracle.cloud_weight_batch(g, cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    cells = oracle.cloud_cells(g, cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert np.all(cells[:, ::3] == -1) and (cells >= 0).sum() > 0.3 * cells.size
+    for i in range(p.shape[0]):
+        acc, n = F32(0.0), 0
+        for c_ in cells[i][cells[i] >= 0]:
+            acc = F32(acc + g.prob[c_])
+            n += 1
+        assert w[i] == (F32(acc / F32(n)) if n > 10 else 0.0), i
 
 
 def test_cloud_weight_constant_grid_and_min_points():
