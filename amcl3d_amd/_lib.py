@@ -121,6 +121,8 @@ SIGNATURES = {
     "a3d_pf_timing": [_vp, _i, _p(_d), _p(_u64), _p(_u64)],
     "a3d_set_tile": [_i],
     "a3d_get_tile": [_p(_i)],
+    "a3d_set_cloud_tile": [_i],
+    "a3d_get_cloud_tile": [_p(_i)],
     "a3d_set_mapping": [_i],
     "a3d_get_mapping": [_p(_i)],
     "a3d_set_order": [_i],
@@ -138,6 +140,12 @@ SIGNATURES = {
     "a3d_fleet_update_staged": [_vp, _vp, _i],
     "a3d_fleet_synchronize": [_vp],
     "a3d_fleet_timing": [_vp, _i, _p(_d), _p(_u64), _p(_u64)],
+    "a3d_fleet_set_layout": [_vp, _vp, _vp],
+    "a3d_fleet_predict_dev": [_vp, _vp, _vp, _vp],
+    "a3d_fleet_update_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "a3d_fleet_resample_dev": [_vp, _vp, _i],
+    "a3d_fleet_capture_begin": [_vp],
+    "a3d_fleet_capture_end": [_vp, _p(_vp)],
     "a3d_server_create": [_p(ServerConfig), _p(_vp)],
     "a3d_server_destroy": [_vp],
     "a3d_server_add_map": [_vp, _vp, _p(_i)],
@@ -245,6 +253,16 @@ def set_tile(particles_per_wave):
 def get_tile():
     v = _i()
     call("a3d_get_tile", ctypes.byref(v))
+    return v.value
+
+
+def set_cloud_tile(points):
+    call("a3d_set_cloud_tile", int(points))
+
+
+def get_cloud_tile():
+    v = _i()
+    call("a3d_get_cloud_tile", ctypes.byref(v))
     return v.value
 
 

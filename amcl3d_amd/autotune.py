@@ -1,10 +1,14 @@
 """Shape autotuner for the cloud-weight kernel.
 
-A shape is (mapping, particles per wavefront, visiting order):
+A shape is (mapping, particles per wavefront, visiting order, points per
+LDS cloud tile):
   * mapping 0 (MAP_PARTICLE_LANES): each lane carries one particle and
-    64 / ppw consecutive cloud points per step (ppw 64, 32 or 16);
+    64 / ppw consecutive cloud points per step (ppw 64, 32 or 16); the
+    workgroup stages the cloud in LDS tiles of 32, 64, 128 or 256 points,
+    one barrier per tile;
   * mapping 1 (MAP_POINT_LANES): the wavefront's 64 lanes are 64 consecutive
-    points, transformed for ppw particles through an LDS tile;
+    points, transformed for ppw particles through an LDS tile (tiles are
+    always 64 points, one per lane);
   * order 0 visits particles in index order, order 1 in a spatial order
     computed on the device before each weight launch (per filter, also in
     fleets and served batches).
@@ -16,8 +20,12 @@ import time
 
 from . import _lib
 
-CANDIDATES = tuple((m, p, o) for o in (1, 0) for m, p in ((0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)))
-FLEET_CANDIDATES = CANDIDATES
+CLOUD_TILES = (32, 64, 128, 256)
+CANDIDATES = tuple([(0, p, o, t) for o in (1, 0) for p in (64, 32, 16) for t in CLOUD_TILES] +
+                   [(1, p, o, 64) for o in (1, 0) for p in (16, 32, 64)])
+# fleets and served batches always visit particles in spatial order (index
+# order loses on every measured workload, and a fleet update is long to time)
+FLEET_CANDIDATES = tuple(c for c in CANDIDATES if c[2] == 1)
 
 
 def set_shape(shape):
@@ -25,10 +33,12 @@ def set_shape(shape):
     _lib.set_tile(shape[1])
     if len(shape) > 2:
         _lib.set_order(shape[2])
+    if len(shape) > 3:
+        _lib.set_cloud_tile(shape[3])
 
 
 def get_shape():
-    return (_lib.get_mapping(), _lib.get_tile(), _lib.get_order())
+    return (_lib.get_mapping(), _lib.get_tile(), _lib.get_order(), _lib.get_cloud_tile())
 
 
 def tune(run, candidates=CANDIDATES, reps=2, prepare=None):
@@ -57,4 +67,6 @@ def shape_name(shape):
     name = "map%d-ppw%d" % tuple(shape[:2])
     if len(shape) > 2:
         name += "-" + ("spatial" if shape[2] else "index")
+    if len(shape) > 3:
+        name += "-tile%d" % shape[3]
     return name

@@ -10,3 +10,10 @@ struct a3d_grid {
   double sensor_dev = 0.0;
   hipStream_t stream = nullptr;
 };
+
+// A captured chain of _dev calls, replayed on the stream it was captured on.
+struct a3d_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+};

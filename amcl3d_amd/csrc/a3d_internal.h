@@ -55,6 +55,11 @@ struct GridDev {
   // 1 when, on every axis, trunc(q * inv_res) equals floor(q / res) for every
   // float q in [0, lim): the cell index needs no window test at all.
   uint32_t clean;
+  // 1 when the prob array with its zero cell, (cells + 1) * 4 bytes, fits in
+  // 2^32 bytes: the cloud weight may load it through a structured buffer
+  // (32-bit byte offsets).  idx64 == 0 and buf32 == 0: 32-bit cell indices
+  // with 64-bit addresses.
+  uint32_t buf32;
   // Half-width of the window around an integer inside which q*inv_res may
   // floor differently from the correctly rounded q/res (max(1e-9, size*2^-48)),
   // and twice that.
@@ -135,9 +140,9 @@ struct OrderDev {
   uint32_t* order;  // n
   uint32_t* hist;   // kOrderKeys + 1
 };
-inline size_t order_bytes(uint32_t n) { return 64 * 8 * 8 + (2 * (size_t)n + kOrderKeys + 1) * 4; }
+constexpr size_t order_bytes(uint32_t n) { return 64 * 8 * 8 + (2 * (size_t)n + kOrderKeys + 1) * 4; }
 // order_bytes rounded up to keep consecutive views 256-byte aligned
-inline size_t order_stride(uint32_t n) { return (order_bytes(n) + 255) & ~(size_t)255; }
+constexpr size_t order_stride(uint32_t n) { return (order_bytes(n) + 255) & ~(size_t)255; }
 inline OrderDev order_view(void* base, uint32_t n) {
   OrderDev o;
   o.part = static_cast<double*>(base);
@@ -200,6 +205,9 @@ int clamp_ppw(int ppw);
 // item per wavefront).
 int item_particles(int mapping, int ppw);
 int current_ppw();
+// Points per LDS cloud tile of the particle-lane kernels (32, 64, 128, 256).
+int clamp_chunk(int points);
+int current_chunk();
 int current_mapping();  // A3D_MAP_PARTICLE_LANES or A3D_MAP_POINT_LANES
 int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 
@@ -210,11 +218,11 @@ int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 //   [12n, 13n) resample float running sum c: its value at the end of every
 //              group of 64 weights (ceil(n / 64) floats)
 //   then FP64 block totals, then 16 floats of per-filter sums (wtp, wtr, wt)
-__host__ __device__ inline size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
-__host__ __device__ inline size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }
-__host__ __device__ inline size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 1) & ~(size_t)1; }
-__host__ __device__ inline size_t sc_sums(uint32_t n) { return sc_tot(n) + 2 * ((size_t)n / 1024 + 2); }
-__host__ __device__ inline size_t scratch_floats(uint32_t n) { return sc_sums(n) + 32; }
+__host__ __device__ constexpr size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
+__host__ __device__ constexpr size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }
+__host__ __device__ constexpr size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 1) & ~(size_t)1; }
+__host__ __device__ constexpr size_t sc_sums(uint32_t n) { return sc_tot(n) + 2 * ((size_t)n / 1024 + 2); }
+__host__ __device__ constexpr size_t scratch_floats(uint32_t n) { return sc_sums(n) + 32; }
 
 // Event pairs around the cloud-weight launches.  Completed pairs are folded
 // into acc_ms and reused as later launches take pairs, so a caller that never

@@ -18,6 +18,7 @@ namespace a3d {
 
 static thread_local std::string g_err;
 static int g_ppw = 16;
+static int g_chunk = 64;  // points per LDS cloud tile (particle-lane kernels)
 static int g_mapping = A3D_MAP_PARTICLE_LANES;
 static int g_order = A3D_ORDER_SPATIAL;
 
@@ -29,6 +30,7 @@ int fail(int code, const std::string& msg) {
 }
 
 int current_ppw() { return g_ppw; }
+int current_chunk() { return g_chunk; }
 int current_mapping() { return g_mapping; }
 int current_order() { return g_order; }
 
@@ -109,6 +111,13 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
   // Test knob: A3D_FORCE_IDX64=1 takes the 64-bit index path on any map.
   const char* force = std::getenv("A3D_FORCE_IDX64");
   if (force && force[0] == '1') g.idx64 = 1u;
+  // A structured buffer forms record * 4 in 32 bits: only a prob array whose
+  // bytes, the zero cell included, fit 2^32 may be read through one.
+  g.buf32 = (!g.idx64 && (g.cells + 1) * 4 <= 0x100000000ull) ? 1u : 0u;
+  // Test knob: A3D_FORCE_IDX32=1 reads 32-bit indices through 64-bit
+  // addresses on any map that has 32-bit indices.
+  const char* force32 = std::getenv("A3D_FORCE_IDX32");
+  if (force32 && force32[0] == '1') g.buf32 = 0u;
 }
 
 }  // namespace a3d
@@ -147,12 +156,6 @@ struct a3d_pf {
     f.n = n;
     return f;
   }
-};
-
-struct a3d_graph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipStream_t stream = nullptr;
 };
 
 
@@ -283,6 +286,19 @@ int a3d_set_tile(int particles_per_wave) {
 int a3d_get_tile(int* particles_per_wave) {
   if (!particles_per_wave) return fail(A3D_ERR_ARG, "a3d_get_tile: null output");
   *particles_per_wave = g_ppw;
+  return A3D_OK;
+}
+
+int a3d_set_cloud_tile(int points) {
+  if (points != 32 && points != 64 && points != 128 && points != 256)
+    return fail(A3D_ERR_ARG, "a3d_set_cloud_tile: points per tile must be 32, 64, 128 or 256");
+  g_chunk = points;
+  return A3D_OK;
+}
+
+int a3d_get_cloud_tile(int* points) {
+  if (!points) return fail(A3D_ERR_ARG, "a3d_get_cloud_tile: null output");
+  *points = g_chunk;
   return A3D_OK;
 }
 

@@ -314,11 +314,13 @@ bool a3d_server::launch(Slot& s) {
   fs.max_n = max_n;
   launch_draws(fs, s.d_seedstep, s.d_seedstep + nf, s.d_u01, st);
   launch_predict(fs, st);
-  // the launch's grid: 64-bit indices if any map of the batch needs them,
-  // clean only if every map is
+  // the launch's grid: the widest index kind any map of the batch needs
+  // (64-bit indices, or 32-bit ones through 64-bit addresses), clean only
+  // if every map is
   GridDev gl = maps[map0]->g;
   for (auto& r : s.reqs) {
     gl.idx64 |= maps[sessions[r.session].map]->g.idx64;
+    gl.buf32 &= maps[sessions[r.session].map]->g.buf32;
     gl.clean &= maps[sessions[r.session].map]->g.clean;
   }
   if (spatial) launch_order_many(s.d_filters, s.d_ord, (uint32_t)nf, max_n, st);  // after predict moved them

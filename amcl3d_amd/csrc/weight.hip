@@ -217,14 +217,23 @@ __device__ void weight_block(const GridDev& g, const FilterDev& f, uint32_t p0, 
 // format, no swizzle, no per-lane stride add.
 constexpr int kBufWord3 = 0x00020000;
 
-// prob[i] for a cell index of the map: 32-bit indices load through a
-// structured buffer of 4-byte records (no 64-bit address arithmetic per lane),
-// 64-bit ones from the global pointer.  Index `cells` is the zero cell.
-template <bool IDX64>
+// How the cloud weight loads prob[i] (the grid's index kind, GridDev):
+//   kIdxBuf  32-bit cell indices through a structured buffer of 4-byte records
+//            (cells + 1 of them): no address arithmetic per lane.  The
+//            hardware forms index * 4 in 32 bits, so only grids whose bytes,
+//            the zero cell included, fit 2^32 take it (GridDev::buf32).
+//   kIdx32   32-bit cell indices, 64-bit addresses (prob + index * 4): grids
+//            of 2^30 to 2^32 - 2 cells.
+//   kIdx64   64-bit cell indices (GridDev::idx64).
+// Index `cells` is the zero cell.  Code for a wider kind reads every grid a
+// narrower kind can (a batch over several maps runs the widest).
+enum IndexKind { kIdxBuf = 0, kIdx32 = 1, kIdx64 = 2 };
+
+template <int IK>
 struct ProbSrc;
 
 template <>
-struct ProbSrc<false> {
+struct ProbSrc<kIdxBuf> {
   using Idx = uint32_t;
   __amdgpu_buffer_rsrc_t r;
   Idx zero;
@@ -235,7 +244,16 @@ struct ProbSrc<false> {
 };
 
 template <>
-struct ProbSrc<true> {
+struct ProbSrc<kIdx32> {
+  using Idx = uint32_t;
+  const float* p;
+  Idx zero;
+  __device__ explicit ProbSrc(const GridDev& g) : p(g.prob), zero((uint32_t)g.cells) {}
+  __device__ float operator()(Idx i) const { return p[(uint64_t)i]; }
+};
+
+template <>
+struct ProbSrc<kIdx64> {
   using Idx = uint64_t;
   const float* p;
   Idx zero;
@@ -285,10 +303,10 @@ __device__ __forceinline__ void rotate(const float (&r)[9], const float4& p, dou
   }
 }
 
-template <bool IDX64>
-__device__ __forceinline__ typename ProbSrc<IDX64>::Idx linear(const GridDev& g, uint32_t ix, uint32_t iy,
+template <int IK>
+__device__ __forceinline__ typename ProbSrc<IK>::Idx linear(const GridDev& g, uint32_t ix, uint32_t iy,
                                                                uint32_t iz) {
-  if constexpr (IDX64) {
+  if constexpr (IK == kIdx64) {
     return (uint64_t)ix + (uint64_t)iy * g.size[0] + (uint64_t)iz * g.step_z;
   } else {
     // (iz * sy + iy) * sx + ix in two 24-bit multiply-adds: the host takes
@@ -331,8 +349,8 @@ enum CellMode { kWindow = 0, kClean = 1, kCleanZero = 2 };
 // whose point is in the map; near (kWindow: some axis needs the exact
 // division, or the lane is slow); and the index to load (the zero cell
 // outside the map).
-template <bool RP0, bool IDX64, int MODE>
-__device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev& g, const ProbSrc<IDX64>& src,
+template <bool RP0, int IK, int MODE>
+__device__ __forceinline__ typename ProbSrc<IK>::Idx cell_fast(const GridDev& g, const ProbSrc<IK>& src,
                                                                   const LanePose& lp, const float4& p, uint64_t& vm,
                                                                   bool& near) {
   double dx, dy, dz;
@@ -352,8 +370,8 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev&
     axis_clean<kZero>(dz, lp.oz, g.lim[2], g.inv_res, inz, iz);
     near = false;
   }
-  const auto lin = linear<IDX64>(g, ix, iy, iz);
-  if constexpr (IDX64) {
+  const auto lin = linear<IK>(g, ix, iy, iz);
+  if constexpr (IK == kIdx64) {
     const bool valid = inx & iny & inz;
     vm = __builtin_amdgcn_ballot_w64(valid);
     return valid ? lin : src.zero;
@@ -364,8 +382,8 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_fast(const GridDev&
 }
 
 // The same with the correctly rounded division on every axis.
-template <bool RP0, bool IDX64>
-__device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_exact(const GridDev& g, const ProbSrc<IDX64>& src,
+template <bool RP0, int IK>
+__device__ __forceinline__ typename ProbSrc<IK>::Idx cell_exact(const GridDev& g, const ProbSrc<IK>& src,
                                                                    const LanePose& lp, const float4& p, bool& valid) {
   double dx, dy, dz;
   rotate<RP0>(lp.r, p, dx, dy, dz);
@@ -375,7 +393,7 @@ __device__ __forceinline__ typename ProbSrc<IDX64>::Idx cell_exact(const GridDev
   axis_exact(dy, lp.oy, g.ext_hi[1], g.res, g.size[1], oky, iy);
   axis_exact(dz, lp.oz, g.ext_hi[2], g.res, g.size[2], okz, iz);
   valid = okx & oky & okz;
-  const auto lin = linear<IDX64>(g, ix, iy, iz);
+  const auto lin = linear<IK>(g, ix, iy, iz);
   return valid ? lin : src.zero;
 }
 
@@ -387,18 +405,31 @@ __device__ __forceinline__ float group_member(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false));
 }
 
-// Points per cloud chunk that a wavefront stages in LDS: one coalesced load
-// per lane and array, then every step reads its point with one LDS read.
-constexpr int kCloudChunk = 64;
+// Points per cloud tile staged in LDS (the tile shape's second dimension,
+// chosen at launch: 32, 64, 128 or 256): the workgroup loads a tile with one
+// coalesced load per thread and array, then every step reads its point with
+// one LDS read.  Per-wavefront staging (work items of different filters in
+// one workgroup) uses tiles of 64.
+constexpr int kMaxChunk = 256;
+constexpr int kWaveChunk = 64;
 
-// The steps of one staged chunk of 64 points (S = 64/Q steps in groups of U).
-template <int Q, bool RP0, bool IDX64, int MODE, typename Acc>
-__device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX64>& src, const LanePose& lp,
-                                            const float4* cl, int po, int& cnt, Acc&& accumulate) {
-  using Idx = typename ProbSrc<IDX64>::Idx;
-  constexpr int U = (Q == 1) ? 8 : 4;  // steps in flight
-  constexpr int S = kCloudChunk / Q;   // steps per chunk
-  static_assert(S % U == 0, "a chunk holds whole groups of steps");
+// Steps whose gathers a wavefront issues together (a group).
+template <int Q>
+constexpr int kGroup = (Q == 1) ? 8 : 4;
+
+// The steps of one staged tile of points: S = points/Q steps, in groups of U
+// (S is a multiple of U for every tile size and Q).  The gathers run one
+// group ahead of the sum: a group's values are added once the next group's
+// cells are computed, so a wavefront computes those cells while its loads
+// are in flight.  pend holds the previous group's values (-0.0f before the
+// first group, which leaves the sum unchanged); the caller adds what is
+// left pending after the last group, before any later point.
+template <int Q, bool RP0, int IK, int MODE, typename Acc>
+__device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IK>& src, const LanePose& lp,
+                                            const float4* cl, int po, int S, int& cnt,
+                                            float (&pend)[kGroup<Q>], Acc&& accumulate) {
+  using Idx = typename ProbSrc<IK>::Idx;
+  constexpr int U = kGroup<Q>;
 #pragma unroll 1
   for (int s0 = 0; s0 < S; s0 += U) {
     Idx idx[U];
@@ -409,7 +440,7 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX6
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       uint64_t vm;
-      idx[u] = cell_fast<RP0, IDX64, MODE>(g, src, lp, pt[u], vm, nr[u]);
+      idx[u] = cell_fast<RP0, IK, MODE>(g, src, lp, pt[u], vm, nr[u]);
       cnt = add_mask_bit(cnt, vm);
     }
     bool any = false;
@@ -425,12 +456,16 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IDX6
           const int k = 4 * ((s0 + u) * Q + po);
           const float4 p = make_float4(vcl[k], vcl[k + 1], vcl[k + 2], vcl[k + 3]);
           bool ve;
-          const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, p, ve);
+          const Idx ie = cell_exact<RP0, IK>(g, src, lp, p, ve);
           cnt += (ve ? 1 : 0) - (idx[u] != src.zero ? 1 : 0);
           idx[u] = ie;
         }
       }
     }
+    // the previous group's values (loaded while these cells were computed),
+    // then this group's loads into the registers they free; the scheduling
+    // barrier keeps the compiler from hoisting the adds (and their wait on
+    // the loads) above the cell arithmetic
     float v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = src(idx[u]);
@@ -445,18 +480,20 @@ __device__ inline float4 cloud_point(const FilterDev& f, uint32_t j) {
 }
 
 // One lane's particle over the whole cloud.  BLOCK: the workgroup's
-// wavefronts (all on the same filter) walk the cloud together: 64-point
-// chunks are staged once per workgroup in LDS (double buffered, one barrier
-// per chunk), so the wavefronts' nearby poses gather the same points at the
-// same time and share cache lines; every wavefront of the workgroup must call
-// this (run = false skips the arithmetic of a wavefront with no active
-// particle).  Otherwise each wavefront stages its own chunks (work-queue
-// kernels, whose wavefronts hold different filters).
-template <int Q, bool RP0, bool IDX64, int MODE, bool BLOCK>
+// wavefronts (all on the same filter) walk the cloud together: tiles of
+// `chunk` points (a multiple of 32, at most kMaxChunk and at most the
+// workgroup's thread count) are staged once per workgroup in LDS (double
+// buffered, one barrier per tile), so the wavefronts' nearby poses gather the
+// same points at the same time and share cache lines; every wavefront of the
+// workgroup must call this (run = false skips the arithmetic of a wavefront
+// with no active particle).  Otherwise each wavefront stages its own tiles of
+// kWaveChunk points (work-queue kernels whose wavefronts hold different
+// filters).
+template <int Q, bool RP0, int IK, int MODE, bool BLOCK>
 __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
-                                           int& cnt, float4* cl, bool run) {
-  using Idx = typename ProbSrc<IDX64>::Idx;
-  const ProbSrc<IDX64> src(g);
+                                           int& cnt, float4* cl, bool run, uint32_t chunk) {
+  using Idx = typename ProbSrc<IK>::Idx;
+  const ProbSrc<IK> src(g);
   const int lane = lane_id();
   const int po = lane % Q;
   const uint32_t m = f.m;
@@ -478,35 +515,43 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
       wsum += group_member<4, 3>(v);
     }
   };
-  const uint32_t mfull = m - m % kCloudChunk;
+  if (!BLOCK) chunk = kWaveChunk;
+  const uint32_t mfull = m - m % chunk;
+  const int steps = (int)(chunk / Q);
+  float pend[kGroup<Q>];
+#pragma unroll
+  for (int u = 0; u < kGroup<Q>; ++u) pend[u] = -0.0f;
   if constexpr (BLOCK) {
     const uint32_t tid = threadIdx.x;
     if (mfull > 0) {
-      if (tid < (uint32_t)kCloudChunk) cl[tid] = cloud_point<RP0>(f, tid);
+      if (tid < chunk) cl[tid] = cloud_point<RP0>(f, tid);
       __syncthreads();
     }
-    for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
-      const float4* cur = cl + ((c0 / kCloudChunk) & 1u) * kCloudChunk;
-      float4* nxt = cl + (((c0 / kCloudChunk) & 1u) ^ 1u) * kCloudChunk;
-      const bool more = c0 + kCloudChunk < mfull;
+    uint32_t buf = 0;
+    for (uint32_t c0 = 0; c0 < mfull; c0 += chunk, buf ^= 1u) {
+      const float4* cur = cl + buf * chunk;
+      float4* nxt = cl + (buf ^ 1u) * chunk;
+      const bool more = c0 + chunk < mfull;
       float4 next = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (more && tid < (uint32_t)kCloudChunk) next = cloud_point<RP0>(f, c0 + kCloudChunk + tid);
-      if (run) lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cur, po, cnt, accumulate);
+      if (more && tid < chunk) next = cloud_point<RP0>(f, c0 + chunk + tid);
+      if (run) lanes_chunk<Q, RP0, IK, MODE>(g, src, lp, cur, po, steps, cnt, pend, accumulate);
       // every wavefront is done with the buffer written next (it read it in
-      // the previous chunk, before the last barrier)
-      if (more && tid < (uint32_t)kCloudChunk) nxt[tid] = next;
+      // the previous tile, before the last barrier)
+      if (more && tid < chunk) nxt[tid] = next;
       __syncthreads();
     }
   } else {
-    for (uint32_t c0 = 0; c0 < mfull; c0 += kCloudChunk) {
-      wave_lds_sync();  // the previous chunk's reads precede these writes
+    for (uint32_t c0 = 0; c0 < mfull; c0 += chunk) {
+      wave_lds_sync();  // the previous tile's reads precede these writes
       cl[lane] = cloud_point<RP0>(f, c0 + lane);
       wave_lds_sync();
-      lanes_chunk<Q, RP0, IDX64, MODE>(g, src, lp, cl, po, cnt, accumulate);
+      lanes_chunk<Q, RP0, IK, MODE>(g, src, lp, cl, po, steps, cnt, pend, accumulate);
     }
   }
   if (!run) return;
-  for (uint32_t c0 = mfull; c0 < m; c0 += Q) {  // the last m mod 64 points
+#pragma unroll
+  for (int u = 0; u < kGroup<Q>; ++u) accumulate(pend[u]);  // the last group of the last tile
+  for (uint32_t c0 = mfull; c0 < m; c0 += Q) {  // the last m mod chunk points
     const uint32_t j = c0 + po;
     const bool pin = j < m;
     const uint32_t js = pin ? j : 0u;
@@ -514,12 +559,12 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
     const float4 p = stage_point<RP0>(pin ? f.cx[js] : __builtin_nanf(""), f.cy[js], f.cz[js]);
     uint64_t vm;
     bool nr;
-    Idx idx = cell_fast<RP0, IDX64, MODE>(g, src, lp, p, vm, nr);
+    Idx idx = cell_fast<RP0, IK, MODE>(g, src, lp, p, vm, nr);
     cnt = add_mask_bit(cnt, vm);
     if (MODE == kWindow && __builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0, 0)) {
       if (nr) {
         bool ve;
-        const Idx ie = cell_exact<RP0, IDX64>(g, src, lp, p, ve);
+        const Idx ie = cell_exact<RP0, IK>(g, src, lp, p, ve);
         cnt += (ve ? 1 : 0) - (idx != src.zero ? 1 : 0);
         idx = ie;
       }
@@ -531,9 +576,9 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
 // CLEAN: the launch's grids are all clean (GridDev::clean).  BLOCK: the
 // workgroup walks the cloud together (every wavefront calls this, count may
 // be 0); cl then holds two chunks per workgroup, else one per wavefront.
-template <int Q, bool IDX64, bool KFILTER, bool CLEAN, bool BLOCK>
+template <int Q, int IK, bool KFILTER, bool CLEAN, bool BLOCK>
 __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, uint32_t count, float* out,
-                             float4* cl) {
+                             float4* cl, uint32_t chunk) {
   const int lane = lane_id();
   const int ps = lane / Q, po = lane % Q;
   const bool has = (uint32_t)ps < count;
@@ -567,20 +612,20 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
     const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;
     if (f.rp0) {
       if (zero)
-        lanes_loop<Q, true, IDX64, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+        lanes_loop<Q, true, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
       else
-        lanes_loop<Q, true, IDX64, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+        lanes_loop<Q, true, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
     } else {
       if (zero)
-        lanes_loop<Q, false, IDX64, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+        lanes_loop<Q, false, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
       else
-        lanes_loop<Q, false, IDX64, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+        lanes_loop<Q, false, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
     }
   } else {
     if (f.rp0)
-      lanes_loop<Q, true, IDX64, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+      lanes_loop<Q, true, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
     else
-      lanes_loop<Q, false, IDX64, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run);
+      lanes_loop<Q, false, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
   }
   if (Q > 1) {
     int c = cnt;
@@ -601,24 +646,25 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
 
 // One filter: the workgroup's wavefronts hold consecutive particles (of the
 // visiting order) and walk the cloud together.
-template <int Q, bool IDX64, bool CLEAN>
-__global__ __launch_bounds__(kBlockThreads) void lanes_single_kernel(GridDev g, FilterDev f) {
+template <int Q, int IK, bool CLEAN>
+__global__ __launch_bounds__(kBlockThreads) void lanes_single_kernel(GridDev g, FilterDev f, uint32_t chunk) {
   constexpr uint32_t P = 64 / Q;
-  __shared__ float4 cloud[2 * kCloudChunk];
+  __shared__ float4 cloud[2 * kMaxChunk];
   const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * kBlockWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
-  weight_lanes<Q, IDX64, true, CLEAN, true>(g, f, p0, count, nullptr, cloud);
+  weight_lanes<Q, IK, true, CLEAN, true>(g, f, p0, count, nullptr, cloud, chunk);
 }
 
-template <int Q, bool IDX64, bool CLEAN>
-__global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out) {
+template <int Q, int IK, bool CLEAN>
+__global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, FilterDev f, float* out,
+                                                                    uint32_t chunk) {
   constexpr uint32_t P = 64 / Q;
-  __shared__ float4 cloud[2 * kCloudChunk];
+  __shared__ float4 cloud[2 * kMaxChunk];
   const uint32_t wave = blockIdx.x * kBlockWaves + (threadIdx.x >> 6);
   const uint32_t p0 = wave * P;
   const uint32_t count = p0 < f.n ? min(P, f.n - p0) : 0u;
-  weight_lanes<Q, IDX64, false, CLEAN, true>(g, f, p0, count, out, cloud);
+  weight_lanes<Q, IK, false, CLEAN, true>(g, f, p0, count, out, cloud, chunk);
 }
 
 // One work item of up to kBlockWaves * P particles of one filter, taken by
@@ -626,29 +672,31 @@ __global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, F
 // wavefronts walk the filter's cloud together.  A filter that names its own
 // grid (a batch over several maps) is weighted on that grid, else on the
 // launch's.
-template <int Q, bool IDX64, bool CLEAN>
-__device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl) {
+template <int Q, int IK, bool CLEAN>
+__device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl,
+                                           uint32_t chunk) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t w = threadIdx.x >> 6, p0 = it.p0 + w * P;
   const uint32_t count = (w * P < it.count) ? min(P, it.count - w * P) : 0u;
   if (f.gdev) {
     const GridDev gf = *f.gdev;
-    weight_lanes<Q, IDX64, true, CLEAN, true>(gf, f, p0, count, nullptr, cl);
+    weight_lanes<Q, IK, true, CLEAN, true>(gf, f, p0, count, nullptr, cl, chunk);
   } else {
-    weight_lanes<Q, IDX64, true, CLEAN, true>(g, f, p0, count, nullptr, cl);
+    weight_lanes<Q, IK, true, CLEAN, true>(g, f, p0, count, nullptr, cl, chunk);
   }
 }
 
-template <int Q, bool IDX64, bool CLEAN>
+template <int Q, int IK, bool CLEAN>
 __global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs,
                                                                     const WorkItem* items, uint32_t nitems,
-                                                                    uint32_t* counter, int persistent) {
-  __shared__ float4 cloud[2 * kCloudChunk];
+                                                                    uint32_t* counter, int persistent,
+                                                                    uint32_t chunk) {
+  __shared__ float4 cloud[2 * kMaxChunk];
   __shared__ uint32_t taken;
   if (!persistent) {
     if (blockIdx.x >= nitems) return;
     const WorkItem it = items[blockIdx.x];
-    lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cloud);
+    lanes_item<Q, IK, CLEAN>(g, fs[it.filter], it, cloud, chunk);
     return;
   }
   for (uint32_t r = 0; r < kXcds; ++r) {
@@ -660,7 +708,7 @@ __global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, c
       __syncthreads();  // every wavefront has read it before the next take
       if (t >= len) break;  // uniform over the workgroup
       const WorkItem it = items[beg + t];
-      lanes_item<Q, IDX64, CLEAN>(g, fs[it.filter], it, cloud);
+      lanes_item<Q, IK, CLEAN>(g, fs[it.filter], it, cloud, chunk);
     }
   }
 }
@@ -781,56 +829,80 @@ void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* item
     weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
 }
 
-// Launches pick the kernel of the grid's index width and, for a clean grid,
-// the one without the window test.
+// The index kind a launch reads its grids with: the widest its maps need.
+inline int index_kind(const GridDev& g) { return g.idx64 ? kIdx64 : (g.buf32 ? kIdxBuf : kIdx32); }
+
+// Launches pick the kernel of the grid's index kind and, for a clean grid,
+// the one without the window test: variant 2 * kind + clean.
+#define A3D_LANES_VARIANT(v, KERNEL, Q, LAUNCH, ...)                                   \
+  switch (v) {                                                                          \
+    case 0: KERNEL<Q, kIdxBuf, false><<<LAUNCH>>>(__VA_ARGS__); break;                  \
+    case 1: KERNEL<Q, kIdxBuf, true><<<LAUNCH>>>(__VA_ARGS__); break;                   \
+    case 2: KERNEL<Q, kIdx32, false><<<LAUNCH>>>(__VA_ARGS__); break;                   \
+    case 3: KERNEL<Q, kIdx32, true><<<LAUNCH>>>(__VA_ARGS__); break;                    \
+    case 4: KERNEL<Q, kIdx64, false><<<LAUNCH>>>(__VA_ARGS__); break;                   \
+    default: KERNEL<Q, kIdx64, true><<<LAUNCH>>>(__VA_ARGS__); break;                   \
+  }
+
+inline int lanes_variant(const GridDev& g) { return 2 * index_kind(g) + (g.clean ? 1 : 0); }
+
 template <int Q>
-void lanes_single_q(const GridDev& g, const FilterDev& f, hipStream_t s) {
+void lanes_single_q(const GridDev& g, const FilterDev& f, uint32_t chunk, hipStream_t s) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
   const uint32_t blocks = (waves + kBlockWaves - 1) / kBlockWaves;
-  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
-  if (k == 0) lanes_single_kernel<Q, false, false><<<blocks, kBlockThreads, 0, s>>>(g, f);
-  else if (k == 1) lanes_single_kernel<Q, false, true><<<blocks, kBlockThreads, 0, s>>>(g, f);
-  else if (k == 2) lanes_single_kernel<Q, true, false><<<blocks, kBlockThreads, 0, s>>>(g, f);
-  else lanes_single_kernel<Q, true, true><<<blocks, kBlockThreads, 0, s>>>(g, f);
+#define A3D_LAUNCH blocks, kBlockThreads, 0, s
+  A3D_LANES_VARIANT(lanes_variant(g), lanes_single_kernel, Q, A3D_LAUNCH, g, f, chunk)
+#undef A3D_LAUNCH
 }
 
 template <int Q>
-void lanes_poses_q(const GridDev& g, const FilterDev& f, float* out, hipStream_t s) {
+void lanes_poses_q(const GridDev& g, const FilterDev& f, float* out, uint32_t chunk, hipStream_t s) {
   constexpr uint32_t P = 64 / Q;
   const uint32_t waves = (f.n + P - 1) / P;
   const uint32_t blocks = (waves + kBlockWaves - 1) / kBlockWaves;
-  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
-  if (k == 0) lanes_poses_kernel<Q, false, false><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
-  else if (k == 1) lanes_poses_kernel<Q, false, true><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
-  else if (k == 2) lanes_poses_kernel<Q, true, false><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
-  else lanes_poses_kernel<Q, true, true><<<blocks, kBlockThreads, 0, s>>>(g, f, out);
+#define A3D_LAUNCH blocks, kBlockThreads, 0, s
+  A3D_LANES_VARIANT(lanes_variant(g), lanes_poses_kernel, Q, A3D_LAUNCH, g, f, out, chunk)
+#undef A3D_LAUNCH
+}
+
+template <int Q>
+int fleet_blocks_per_cu(int v) {
+  int per_cu = 1;
+  switch (v) {
+    case 0: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdxBuf, false>,
+                                                               kBlockThreads, 0); break;
+    case 1: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdxBuf, true>,
+                                                               kBlockThreads, 0); break;
+    case 2: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdx32, false>,
+                                                               kBlockThreads, 0); break;
+    case 3: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdx32, true>,
+                                                               kBlockThreads, 0); break;
+    case 4: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdx64, false>,
+                                                               kBlockThreads, 0); break;
+    default: (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanes_fleet_kernel<Q, kIdx64, true>,
+                                                                kBlockThreads, 0); break;
+  }
+  return per_cu;
 }
 
 // g.clean must hold for every grid the filters name (the caller clears it
-// otherwise).
+// otherwise), and g's index kind must be at least as wide as each grid's.
 template <int Q>
 void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items, uint32_t nitems, uint32_t* counter,
-                   int persistent, hipStream_t s) {
-  const int k = (g.idx64 ? 2 : 0) + (g.clean ? 1 : 0);
+                   int persistent, uint32_t chunk, hipStream_t s) {
+  const int v = lanes_variant(g);
   uint32_t blocks;
   if (persistent) {
-    static int cus = 0, per_cu[4] = {0, 0, 0, 0};
+    static int cus = 0, per_cu[6] = {0, 0, 0, 0, 0, 0};
     if (!cus) {
       int dev = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], lanes_fleet_kernel<Q, false, false>,
-                                                         kBlockThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], lanes_fleet_kernel<Q, false, true>,
-                                                         kBlockThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], lanes_fleet_kernel<Q, true, false>,
-                                                         kBlockThreads, 0);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[3], lanes_fleet_kernel<Q, true, true>,
-                                                         kBlockThreads, 0);
+      for (int k = 0; k < 6; ++k) per_cu[k] = fleet_blocks_per_cu<Q>(k);
       if (cus < 1) cus = 256;
     }
-    blocks = (uint32_t)(cus * std::max(1, per_cu[k]));
+    blocks = (uint32_t)(cus * std::max(1, per_cu[v]));
     if (blocks > nitems) blocks = nitems;  // one item per workgroup
     if (blocks == 0) blocks = 1;
     zero_u32_kernel<<<1, 64, 0, s>>>(counter, kXcds);
@@ -839,11 +911,12 @@ void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items,
   }
   if (blocks == 0) return;
   const dim3 b(blocks), t(kBlockThreads);
-  if (k == 0) lanes_fleet_kernel<Q, false, false><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else if (k == 1) lanes_fleet_kernel<Q, false, true><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else if (k == 2) lanes_fleet_kernel<Q, true, false><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
-  else lanes_fleet_kernel<Q, true, true><<<b, t, 0, s>>>(g, fs, items, nitems, counter, persistent);
+#define A3D_LAUNCH b, t, 0, s
+  A3D_LANES_VARIANT(v, lanes_fleet_kernel, Q, A3D_LAUNCH, g, fs, items, nitems, counter, persistent, chunk)
+#undef A3D_LAUNCH
 }
+
+#undef A3D_LANES_VARIANT
 
 }  // namespace
 
@@ -858,21 +931,29 @@ int clamp_ppw(int ppw) {
   return 64;
 }
 
+int clamp_chunk(int points) {
+  if (points <= 32) return 32;
+  if (points <= 64) return 64;
+  if (points <= 128) return 128;
+  return kMaxChunk;
+}
+
 void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
                    uint32_t* d_counter, int ppw, int persistent, hipStream_t s) {
   ppw = clamp_ppw(ppw);
   if (current_mapping() == 0) {  // lanes = particles, ppw particles per wavefront
+    const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
     if (!fs.arr) {
       if (fs.one.n == 0) return;
-      if (ppw == 64) lanes_single_q<1>(g, fs.one, s);
-      else if (ppw == 32) lanes_single_q<2>(g, fs.one, s);
-      else lanes_single_q<4>(g, fs.one, s);
+      if (ppw == 64) lanes_single_q<1>(g, fs.one, chunk, s);
+      else if (ppw == 32) lanes_single_q<2>(g, fs.one, chunk, s);
+      else lanes_single_q<4>(g, fs.one, chunk, s);
       return;
     }
     if (nitems == 0) return;
-    if (ppw == 64) lanes_fleet_q<1>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
-    else if (ppw == 32) lanes_fleet_q<2>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
-    else lanes_fleet_q<4>(g, fs.arr, d_items, nitems, d_counter, persistent, s);
+    if (ppw == 64) lanes_fleet_q<1>(g, fs.arr, d_items, nitems, d_counter, persistent, chunk, s);
+    else if (ppw == 32) lanes_fleet_q<2>(g, fs.arr, d_items, nitems, d_counter, persistent, chunk, s);
+    else lanes_fleet_q<4>(g, fs.arr, d_items, nitems, d_counter, persistent, chunk, s);
     return;
   }
   if (!fs.arr) {
@@ -907,9 +988,10 @@ void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, con
   f.rp0 = (roll == 0.0f && pitch == 0.0f) ? 1u : 0u;
   ppw = clamp_ppw(ppw);
   if (current_mapping() == 0) {
-    if (ppw == 64) lanes_poses_q<1>(g, f, out, s);
-    else if (ppw == 32) lanes_poses_q<2>(g, f, out, s);
-    else lanes_poses_q<4>(g, f, out, s);
+    const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
+    if (ppw == 64) lanes_poses_q<1>(g, f, out, chunk, s);
+    else if (ppw == 32) lanes_poses_q<2>(g, f, out, chunk, s);
+    else lanes_poses_q<4>(g, f, out, chunk, s);
     return;
   }
   if (ppw == 16) launch_poses_pb<16>(g, f, out, s);

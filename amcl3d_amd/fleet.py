@@ -78,6 +78,35 @@ class Fleet:
     def resample(self, u01, mode=_lib.RESAMPLE_REFERENCE):
         _lib.call("a3d_fleet_resample", self._h, _lib.f32(u01), int(mode))
 
+    # device-parameter forms (capturable): see a3d_fleet_set_layout
+    def set_layout(self, m, nb):
+        self.layout_m = _lib.u32(m)
+        self.layout_nb = _lib.u32(nb)
+        _lib.call("a3d_fleet_set_layout", self._h, self.layout_m, self.layout_nb)
+        self.evals = int(np.sum(self.counts.astype(np.int64) * self.layout_m.astype(np.int64)))
+
+    def predict_dev(self, d_mods, d_deltas, d_z4):
+        """Device pointers: 4 doubles per filter (twice), 4 normals per particle."""
+        _lib.call("a3d_fleet_predict_dev", self._h, d_mods, d_deltas, d_z4)
+
+    def update_dev(self, grid, d_x, d_y, d_z, d_ranges, d_params):
+        """Device pointers: clouds and ranges concatenated in the layout's
+        filter order; params {alpha, sigma, roll, pitch} doubles per filter."""
+        _lib.call("a3d_fleet_update_dev", self._h, grid.handle, d_x, d_y, d_z, d_ranges, d_params)
+
+    def resample_dev(self, d_u01, mode=_lib.RESAMPLE_REFERENCE):
+        _lib.call("a3d_fleet_resample_dev", self._h, d_u01, int(mode))
+
+    def capture_begin(self):
+        _lib.call("a3d_fleet_capture_begin", self._h)
+
+    def capture_end(self):
+        from .particle_filter import Graph
+
+        g = ctypes.c_void_p()
+        _lib.call("a3d_fleet_capture_end", self._h, ctypes.byref(g))
+        return Graph(g.value)
+
     def timing(self, reset=False):
         ms, n, e = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
         _lib.call("a3d_fleet_timing", self._h, 1 if reset else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(e))

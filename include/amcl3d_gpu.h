@@ -208,6 +208,12 @@ int a3d_pf_timing(a3d_pf* pf, int reset, double* weight_ms_avg, uint64_t* launch
 enum { A3D_MAP_PARTICLE_LANES = 0, A3D_MAP_POINT_LANES = 1 };
 int a3d_set_tile(int particles_per_wave);
 int a3d_get_tile(int* particles_per_wave);
+/* Points of the cloud staged per LDS tile by the A3D_MAP_PARTICLE_LANES
+ * kernels (32, 64, 128 or 256; default 64): the workgroup's wavefronts walk
+ * the cloud one tile at a time, one barrier per tile.  Point-lane kernels
+ * always stage 64 (one point per lane). */
+int a3d_set_cloud_tile(int points);
+int a3d_get_cloud_tile(int* points);
 int a3d_set_mapping(int mapping);
 int a3d_get_mapping(int* mapping);
 /* Order in which the cloud-weight kernel visits a filter's particles (the
@@ -247,6 +253,36 @@ int a3d_fleet_update_staged(a3d_fleet* fl, const a3d_grid* g, int persistent);
 int a3d_fleet_synchronize(a3d_fleet* fl);
 /* Cloud-weight kernel timing for the fleet (as a3d_pf_timing). */
 int a3d_fleet_timing(a3d_fleet* fl, int reset, double* weight_ms_avg, uint64_t* launches, uint64_t* evals);
+
+/* Device-parameter forms: every per-step input is a device array, so a step
+ * only enqueues kernels on the fleet's stream (no host synchronisation) and
+ * one predict -> update -> resample of all filters can be captured in a HIP
+ * graph and replayed; the graph reads the arrays' contents at each replay.
+ * Each is ParticleFilter::predict / update / resample (ParticleFilter.cpp:
+ * 97-222) for every filter, as a3d_fleet_predict / _update / _resample.
+ *
+ * a3d_fleet_set_layout fixes the cloud size m[f] and beacon count nb[f] of
+ * every filter for the _dev calls and cuts the work list for the current
+ * kernel shape (host call, synchronises).  a3d_fleet_init and
+ * a3d_fleet_predict (host forms) and a3d_fleet_stage_update clear it.
+ *   predict_dev: d_mods, d_deltas 4 doubles per filter; d_z4 4 normals per
+ *                particle, filters concatenated.
+ *   update_dev:  d_x, d_y, d_z the filters' clouds concatenated in filter
+ *                order (m[f] points each); d_ranges nb[f] x {r, ax, ay, az}
+ *                each, concatenated; d_params {alpha, sigma, roll, pitch}
+ *                per filter (doubles; roll and pitch are narrowed to float
+ *                as computeCloudWeight's parameters).
+ *   resample_dev: d_u01 one uniform per filter. */
+int a3d_fleet_set_layout(a3d_fleet* fl, const uint32_t* m, const uint32_t* nb);
+int a3d_fleet_predict_dev(a3d_fleet* fl, const double* d_mods, const double* d_deltas, const float* d_z4);
+int a3d_fleet_update_dev(a3d_fleet* fl, const a3d_grid* g, const float* d_x, const float* d_y, const float* d_z,
+                         const float* d_ranges, const double* d_params);
+int a3d_fleet_resample_dev(a3d_fleet* fl, const float* d_u01, int mode);
+/* Capture of _dev calls on the fleet's stream into a graph (a3d_graph_launch
+ * replays it; a3d_graph_destroy frees it).  Host forms are refused while
+ * capturing. */
+int a3d_fleet_capture_begin(a3d_fleet* fl);
+int a3d_fleet_capture_end(a3d_fleet* fl, a3d_graph** out);
 
 /* ------------------------------------------------------------------ Serving */
 /* A request loop that keeps every session's particles, weights and RNG state

@@ -3,9 +3,10 @@ at 0.05 m (2000 x 2000 x 600 = 2.4e9 cells, 9.6 GB of prob), built on the
 device.
 
 The map has fewer than 2^32 - 1 cells, sx < 2^24 and sy * sz < 2^24, so the
-cloud weight gathers it through 32-bit cell indices (a structured buffer of
-4-byte records).  Poses on the upper storeys land points in cells past index
-2^30 and 2^31, at byte offsets past 4 GiB and 8 GiB.  The device weights, a
+cloud weight gathers it through 32-bit cell indices; its 9.6 GB are more than
+a structured buffer's 32-bit byte offsets reach, so the gathers take 64-bit
+addresses.  Poses on the upper storeys land points in cells past index 2^30
+and 2^31, at byte offsets past 4 GiB and 8 GiB.  The device weights, a
 captured predict -> update -> resample step and a 100,000-particle update
 are checked against the oracle on the grid read back from the device, and the
 grid itself against the oracle's nearest distances on sampled cells, most of
@@ -174,3 +175,40 @@ def test_large_filter_top_storey(gpu, building):
     assert abs(got[:, 4].astype(np.float64).sum() - 1.0) < 1e-3
     idx = pf.resample(0.37, gpu.RESAMPLE_REFERENCE, return_indices=True)
     assert np.array_equal(idx, oracle.resample_indices(got[:, 4], 0.37, 0))
+
+
+def test_fleet_upper_storeys(gpu, building):
+    """Eight filters on storeys 4 to 9 in one persistent fleet launch (the
+    batched kernel through the same 64-bit addresses): every filter's updated
+    particles and mean bit-exact against the oracle."""
+    from amcl3d_amd.fleet import Fleet
+
+    bm, g, og = building["bm"], building["grid"], building["og"]
+    rng = np.random.default_rng(37)
+    F = 8
+    counts = rng.integers(200, 1500, F).astype(np.uint32)
+    parts, clouds, ranges = [], [], []
+    for f in range(F):
+        pose, view = _pose(bm, STOREY_Z[f % len(STOREY_Z)], rng)
+        clouds.append(W.sensor_cloud(view, pose, int(rng.integers(2000, 8000)), 0.03, rng))
+        anchors = rng.uniform(bm.min + 1, bm.max - 1, (3, 3))
+        r = np.linalg.norm(anchors - pose[:3], axis=1) + rng.normal(0, 0.1, 3)
+        ranges.append(np.concatenate([r[:, None], anchors], 1).astype(F32))
+        parts.append(W.particles_around(pose, int(counts[f]), rng))
+    fl = Fleet(counts)
+    try:
+        fl.set_particles(np.concatenate(parts))
+        fl.stage_update(clouds, ranges, 0.8, 0.1)
+        _say("fleet: %d filters on the upper storeys" % F)
+        fl.update_staged(g)
+        fl.synchronize()
+        got, means = fl.particles(), fl.means()
+        off = 0
+        for f in range(F):
+            n = int(counts[f])
+            op, om = oracle.pf_update(parts[f], og, clouds[f], ranges[f], 0.8, 0.1)
+            assert np.array_equal(got[off:off + n].view(np.int32), op.view(np.int32)), f
+            assert np.array_equal(means[f].view(np.int32), om.view(np.int32)), f
+            off += n
+    finally:
+        fl.close()

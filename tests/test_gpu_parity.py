@@ -141,17 +141,48 @@ def test_grid_file_round_trip(gpu, room, tmp_path):
 
 
 # --------------------------------------------------------- computeCloudWeight
-@pytest.fixture(params=[(0, 64), (0, 32), (0, 16), (1, 16), (1, 32), (1, 64)],
-                ids=lambda mt: "map%d-ppw%d" % mt)
+# (mapping, particles per wavefront, points per LDS cloud tile): every ppw and
+# every tile size of the particle-lane kernels, and the point-lane kernels
+SHAPES = [(0, 64, 64), (0, 64, 256), (0, 32, 32), (0, 32, 128), (0, 16, 32), (0, 16, 64), (0, 16, 128),
+          (0, 16, 256), (1, 16, 64), (1, 32, 64), (1, 64, 64)]
+
+
+@pytest.fixture(params=SHAPES, ids=lambda mt: "map%d-ppw%d-tile%d" % mt)
 def shape(gpu, request):
-    """Every kernel shape (mapping, particles per wavefront) must give the same bits."""
-    m0, t0 = gpu.get_mapping(), gpu.get_tile()
+    """Every kernel shape (mapping, particles per wavefront, cloud tile) must
+    give the same bits."""
+    m0, t0, c0 = gpu.get_mapping(), gpu.get_tile(), gpu.get_cloud_tile()
     gpu.set_mapping(request.param[0])
     gpu.set_tile(request.param[1])
-    _say("cloud-weight shape mapping=%d ppw=%d" % request.param)
+    gpu.set_cloud_tile(request.param[2])
+    _say("cloud-weight shape mapping=%d ppw=%d tile=%d" % request.param)
     yield request.param
     gpu.set_mapping(m0)
     gpu.set_tile(t0)
+    gpu.set_cloud_tile(c0)
+
+
+@pytest.mark.parametrize("tile", [32, 64, 128, 256])
+def test_cloud_weights_ragged_clouds_every_tile(gpu, room, tile):
+    """Clouds shorter than a tile, one point either side of a tile boundary,
+    and several tiles plus a remainder: every tile size gives the oracle's
+    bits (the points past the last whole tile are read outside LDS)."""
+    s, p = room["step"], room["particles"][:300]
+    saved = gpu.get_mapping(), gpu.get_tile(), gpu.get_cloud_tile()
+    try:
+        gpu.set_mapping(gpu.MAP_PARTICLE_LANES)
+        gpu.set_cloud_tile(tile)
+        for ppw in (64, 16):
+            gpu.set_tile(ppw)
+            for m in (1, 11, 12, tile - 1, tile, tile + 1, 2 * tile + 7, 1999):
+                cloud = s.cloud[:m]
+                w = room["grid"].cloud_weights(cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+                ow = oracle.cloud_weight_batch(room["ogrid"], cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+                assert np.array_equal(w.view(np.int32), ow.view(np.int32)), (tile, ppw, m)
+    finally:
+        gpu.set_mapping(saved[0])
+        gpu.set_tile(saved[1])
+        gpu.set_cloud_tile(saved[2])
 
 
 def test_cloud_weights_bit_exact(gpu, room, shape):
@@ -203,6 +234,18 @@ def test_cloud_weights_roll_pitch_and_edges(gpu, room, shape):
 
 def test_cloud_weights_64bit_index_path(gpu, room, shape, monkeypatch):
     monkeypatch.setenv("A3D_FORCE_IDX64", "1")
+    bm = room["map"]
+    g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
+    s, p = room["step"], room["particles"]
+    w = g.cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
+def test_cloud_weights_32bit_index_64bit_address_path(gpu, room, shape, monkeypatch):
+    """The gather of grids of 2^30 to 2^32 - 2 cells (32-bit cell indices,
+    64-bit addresses), forced on the room map."""
+    monkeypatch.setenv("A3D_FORCE_IDX32", "1")
     bm = room["map"]
     g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
     s, p = room["step"], room["particles"]
