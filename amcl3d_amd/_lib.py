@@ -113,6 +113,7 @@ SIGNATURES = {
     "a3d_pf_predict_dev": [_vp, _vp, _vp, _vp],
     "a3d_pf_update_dev": [_vp, _vp, _vp, _vp, _vp, _sz, _vp, _sz, _d, _d, _d, _d],
     "a3d_pf_resample_dev": [_vp, _f, _i],
+    "a3d_pf_update_noalloc": [_vp, _vp, _sz, _vp, _vp, _vp, _sz, _vp, _sz, _d, _d, _d, _d, _vp, _vp],
     "a3d_pf_synchronize": [_vp],
     "a3d_pf_capture_begin": [_vp],
     "a3d_pf_capture_end": [_vp, _p(_vp)],

@@ -188,6 +188,21 @@ int a3d_pf_update_dev(a3d_pf* pf, const a3d_grid* g, const float* d_x, const flo
                       size_t m, const float* d_ranges, size_t nb, double alpha, double sigma, double roll,
                       double pitch);
 int a3d_pf_resample_dev(a3d_pf* pf, float u01, int mode);
+/* No counterpart in the reference: a3d_pf_update_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  It allocates nothing, not even on its
+ * first call: the particles are the caller's, seven device arrays of n floats
+ * in one block (x, y, z, a, w, wp, wr; field k at d_particles + k*n), updated
+ * in place as ParticleFilter::update (ParticleFilter.cpp:121-196) does, and
+ * d_mean (4 floats) receives the mean.  Its working memory is a fixed static
+ * device array for at most 131072 particles (about 8 MB per device, held
+ * until the process ends); larger n fails with A3D_ERR_ARG.  Enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream) without waiting.  Every
+ * call on one device shares that array, so calls that overlap on a device
+ * race: serialise them (one stream, or wait between calls). */
+int a3d_pf_update_noalloc(const a3d_grid* g, float* d_particles, size_t n, const float* d_x, const float* d_y,
+                          const float* d_z, size_t m, const float* d_ranges, size_t nb, double alpha, double sigma,
+                          double roll, double pitch, float* d_mean, void* stream);
 int a3d_pf_synchronize(a3d_pf* pf);
 /* Captures the _dev calls made between begin and end into a HIP graph. */
 int a3d_pf_capture_begin(a3d_pf* pf);
