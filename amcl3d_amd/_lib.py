@@ -122,6 +122,8 @@ SIGNATURES = {
     "a3d_pf_timing": [_vp, _i, _p(_d), _p(_u64), _p(_u64)],
     "a3d_set_tile": [_i],
     "a3d_get_tile": [_p(_i)],
+    "a3d_set_deal": [_i],
+    "a3d_get_deal": [_p(_i)],
     "a3d_set_cloud_tile": [_i],
     "a3d_get_cloud_tile": [_p(_i)],
     "a3d_set_mapping": [_i],
@@ -259,6 +261,16 @@ def get_tile():
 
 def set_cloud_tile(points):
     call("a3d_set_cloud_tile", int(points))
+
+
+def set_deal(run):
+    call("a3d_set_deal", int(run))
+
+
+def get_deal():
+    v = _i()
+    call("a3d_get_deal", ctypes.byref(v))
+    return v.value
 
 
 def get_cloud_tile():

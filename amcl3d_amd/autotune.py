@@ -11,7 +11,10 @@ LDS cloud tile):
     always 64 points, one per lane);
   * order 0 visits particles in index order, order 1 in a spatial order
     computed on the device before each weight launch (per filter, also in
-    fleets and served batches).
+    fleets and served batches);
+  * persistent launches (fleets, served batches) add a fifth element, how the
+    work list is dealt to the XCDs: 0 contiguous ranges, k runs of k blocks
+    round-robin.
 Every shape gives bit-identical results, so the choice is purely a timing
 one: run the caller's workload once per candidate (after one untimed warm-up)
 and keep the fastest.
@@ -25,7 +28,9 @@ CANDIDATES = tuple([(0, p, o, t) for o in (1, 0) for p in (64, 32, 16) for t in 
                    [(1, p, o, 64) for o in (1, 0) for p in (16, 32, 64)])
 # fleets and served batches always visit particles in spatial order (index
 # order loses on every measured workload, and a fleet update is long to time)
-FLEET_CANDIDATES = tuple(c for c in CANDIDATES if c[2] == 1)
+LOOP_CANDIDATES = tuple(c for c in CANDIDATES if c[2] == 1)
+DEALS = (0, 4)
+FLEET_CANDIDATES = tuple(c + (d,) for d in DEALS for c in LOOP_CANDIDATES)
 
 
 def set_shape(shape):
@@ -35,10 +40,12 @@ def set_shape(shape):
         _lib.set_order(shape[2])
     if len(shape) > 3:
         _lib.set_cloud_tile(shape[3])
+    if len(shape) > 4:
+        _lib.set_deal(shape[4])
 
 
 def get_shape():
-    return (_lib.get_mapping(), _lib.get_tile(), _lib.get_order(), _lib.get_cloud_tile())
+    return (_lib.get_mapping(), _lib.get_tile(), _lib.get_order(), _lib.get_cloud_tile(), _lib.get_deal())
 
 
 def tune(run, candidates=CANDIDATES, reps=2, prepare=None):
@@ -69,4 +76,6 @@ def shape_name(shape):
         name += "-" + ("spatial" if shape[2] else "index")
     if len(shape) > 3:
         name += "-tile%d" % shape[3]
+    if len(shape) > 4:
+        name += "-deal%d" % shape[4]
     return name

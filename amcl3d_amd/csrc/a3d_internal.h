@@ -198,6 +198,11 @@ int voxel_grid_device(const void* d_pts, size_t n, size_t stride, float leaf, fl
 // filter to d_u01 (may be null).
 void launch_draws(const Filters& fs, const uint64_t* d_seed, const uint64_t* d_step, float* d_u01, hipStream_t s);
 
+// A filter's stream, and its resample with the uniform in device memory
+// (capi.cpp; for the node glue).
+hipStream_t pf_stream(const a3d_pf* p);
+int pf_resample_device_u01(a3d_pf* p, const float* d_u01);
+
 // Cloud-weight kernel tile sizes.
 int clamp_ppw(int ppw);
 // Particles per work item of a work list for the given mapping and ppw (the
@@ -208,6 +213,9 @@ int current_ppw();
 // Points per LDS cloud tile of the particle-lane kernels (32, 64, 128, 256).
 int clamp_chunk(int points);
 int current_chunk();
+// Work-list dealing of persistent (fleet, served) launches: 0 contiguous
+// ranges per XCD, k > 0 runs of k items dealt round-robin.
+int current_deal();
 int current_mapping();  // A3D_MAP_PARTICLE_LANES or A3D_MAP_POINT_LANES
 int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 
@@ -217,12 +225,21 @@ int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 //   [10n, 12n) resample FP64 prefix (n doubles, 8-byte aligned)
 //   [12n, 13n) resample float running sum c: its value at the end of every
 //              group of 64 weights (ceil(n / 64) floats)
-//   then FP64 block totals, then 16 floats of per-filter sums (wtp, wtr, wt)
+//   then FP64 block totals, then 32 floats of per-filter sums (wtp, wtr, wt,
+//   c, the irregular-weights flag), then the tile records of the tiled exact
+//   sums (kTiledSums sums of tiled_tiles(n) records of 16 floats, then their
+//   sub-tile records, 16 per tile of 8 floats each)
 __host__ __device__ constexpr size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
 __host__ __device__ constexpr size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }
 __host__ __device__ constexpr size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 1) & ~(size_t)1; }
 __host__ __device__ constexpr size_t sc_sums(uint32_t n) { return sc_tot(n) + 2 * ((size_t)n / 1024 + 2); }
-__host__ __device__ constexpr size_t scratch_floats(uint32_t n) { return sc_sums(n) + 32; }
+constexpr uint32_t kTiledTile = 1024;  // terms per tile of a tiled exact sum
+constexpr uint32_t kTiledSums = 5;     // sums one tiled launch computes at most
+__host__ __device__ constexpr size_t tiled_tiles(uint32_t n) { return ((size_t)n + kTiledTile - 1) / kTiledTile; }
+__host__ __device__ constexpr size_t sc_tiles(uint32_t n) { return (sc_sums(n) + 32 + 7) & ~(size_t)7; }
+__host__ __device__ constexpr size_t scratch_floats(uint32_t n) {
+  return sc_tiles(n) + (16 + 16 * 8) * kTiledSums * tiled_tiles(n);
+}
 
 // Event pairs around the cloud-weight launches.  Completed pairs are folded
 // into acc_ms and reused as later launches take pairs, so a caller that never

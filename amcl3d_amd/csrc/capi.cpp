@@ -19,6 +19,7 @@ namespace a3d {
 static thread_local std::string g_err;
 static int g_ppw = 16;
 static int g_chunk = 64;  // points per LDS cloud tile (particle-lane kernels)
+static int g_deal = 0;    // work-list dealing of persistent launches
 static int g_mapping = A3D_MAP_PARTICLE_LANES;
 static int g_order = A3D_ORDER_SPATIAL;
 
@@ -31,6 +32,7 @@ int fail(int code, const std::string& msg) {
 
 int current_ppw() { return g_ppw; }
 int current_chunk() { return g_chunk; }
+int current_deal() { return g_deal; }
 int current_mapping() { return g_mapping; }
 int current_order() { return g_order; }
 
@@ -299,6 +301,18 @@ int a3d_set_cloud_tile(int points) {
 int a3d_get_cloud_tile(int* points) {
   if (!points) return fail(A3D_ERR_ARG, "a3d_get_cloud_tile: null output");
   *points = g_chunk;
+  return A3D_OK;
+}
+
+int a3d_set_deal(int run) {
+  if (run < 0 || run > 4096) return fail(A3D_ERR_ARG, "a3d_set_deal: run length must be 0 (contiguous) to 4096");
+  g_deal = run;
+  return A3D_OK;
+}
+
+int a3d_get_deal(int* run) {
+  if (!run) return fail(A3D_ERR_ARG, "a3d_get_deal: null output");
+  *run = g_deal;
   return A3D_OK;
 }
 
@@ -813,6 +827,28 @@ int a3d_pf_update(a3d_pf* p, const a3d_grid* g, const float* x, const float* y, 
   A3D_HIP(hipStreamSynchronize(p->stream));
   return A3D_OK;
 }
+
+}  // extern "C"
+
+namespace a3d {
+hipStream_t pf_stream(const a3d_pf* p) { return p->stream; }
+
+// a3d_pf_resample_dev with the uniform read from device memory when the
+// resample runs (the node's draws leave it there), reference mode.
+int pf_resample_device_u01(a3d_pf* p, const float* d_u01) {
+  if (!p || !d_u01) return fail(A3D_ERR_ARG, "resample: null argument");
+  if (!p->n) return fail(A3D_ERR_STATE, "resample: filter has no particles (call init)");
+  Filters fs = single(p->dev());
+  fs.one.idx_out = p->idx;
+  fs.u01 = d_u01;
+  launch_resample(fs, A3D_RESAMPLE_REFERENCE, p->prefix_valid, p->stream);
+  p->prefix_valid = false;
+  A3D_HIP(hipGetLastError());
+  return A3D_OK;
+}
+}  // namespace a3d
+
+extern "C" {
 
 int a3d_pf_resample_dev(a3d_pf* p, float u01, int mode) {
   if (!p) return fail(A3D_ERR_ARG, "a3d_pf_resample: null filter");

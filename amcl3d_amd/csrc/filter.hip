@@ -32,7 +32,8 @@ enum ChainWhich {
   kTailTotals = 0,   // wtp, wtr over the in-map wp, wr     -> sums[0], sums[1]
   kWeightTotal = 1,  // wt over w                           -> sums[2]
   kPrefixOnly = 3,   // c over w                            -> c
-  kInitMean = 4      // mean x, y, z, a                     -> mean[0..3]
+  kInitMean = 4,     // mean x, y, z, a                     -> mean[0..3]
+  kMeanPrefix = 5    // mean x, y, z, a and c (tiled sums)  -> mean[0..3], c
 };
 
 __device__ inline const FilterDev& fsel(const Filters& fs, uint32_t i) { return fs.arr ? fs.arr[i] : fs.one; }
@@ -241,7 +242,8 @@ __device__ __forceinline__ ParityFn row_inclusive_fn(ParityFn f) {
 // rows of 16 lanes by DPP shifts, the four rows of a wavefront by row
 // broadcasts (lane 15 into rows 1 and 3, then lane 31 into rows 2 and 3),
 // then the 16 wavefront totals, composed by wavefront 0 in one row.
-__device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, ExactLds& L) {
+template <typename Lds>
+__device__ __forceinline__ ParityFn block_exclusive_fn(ParityFn f, Lds& L) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int kW = kChainThreads / 64;
   static_assert(kW == 16, "the wavefront totals are composed in one row of 16 lanes");
@@ -311,23 +313,24 @@ __device__ __forceinline__ float serial_add(float a, const float* stage, uint32_
   return a;
 }
 
-// s + x[pos] + ... + x[n-1] in order: lane 0 adds singly up to the next
-// checkpoint-group boundary, then the serial chain (chain_body) runs the rest.
-__device__ __forceinline__ float serial_rest(float s, uint32_t pos, uint32_t n, const float* __restrict__ x,
-                                             float* prefix, ExactLds& L) {
+// s + x[pos] + ... + x[end-1] in order (n: the sum's length, for its last
+// checkpoint): lane 0 adds singly up to the next checkpoint-group boundary,
+// then the serial chain (chain_body) runs the rest.
+__device__ __forceinline__ float serial_rest(float s, uint32_t pos, uint32_t end, uint32_t n,
+                                             const float* __restrict__ x, float* prefix, ExactLds& L) {
   static_assert(ChainLds<1, 0>::kFloats <= kExactTile, "the chain's staging fits the tile buffer");
-  const uint32_t pa = min(n, (pos + kGroup - 1) / kGroup * kGroup);
+  const uint32_t pa = min(end, (pos + kGroup - 1) / kGroup * kGroup);
   if (threadIdx.x == 0) L.s = serial_add(s, x + pos, pos, pa - pos, n, prefix);
   __syncthreads();
   s = L.s;
   __syncthreads();
-  if (pa < n) {
+  if (pa < end) {
     const float* const src[1] = {x + pa};
     float* const dst[1] = {&L.s};
     if (prefix)
-      chain_body<1, 0>(n - pa, src, dst, prefix + pa / kGroup, L.stage, s);
+      chain_body<1, 0>(end - pa, src, dst, prefix + pa / kGroup, L.stage, s);
     else
-      chain_body<1, -1>(n - pa, src, dst, nullptr, L.stage, s);
+      chain_body<1, -1>(end - pa, src, dst, nullptr, L.stage, s);
     __syncthreads();
     s = L.s;
     __syncthreads();
@@ -335,34 +338,35 @@ __device__ __forceinline__ float serial_rest(float s, uint32_t pos, uint32_t n, 
   return s;
 }
 
-// The sequential float sum of x[0..n) from +0, exactly; with prefix, the
-// running sum after element i is stored at prefix[i / kGroup] at the end of
-// every group of kGroup elements and after the last element.  Called by every
-// thread of a kChainThreads workgroup; returns the sum in every thread.
-__device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefix, ExactLds& L) {
+// The sequential float sum s + x[pos] + ... + x[end-1], exactly (n: the
+// whole sum's length); with prefix, the running sum after element i is
+// stored at prefix[i / kGroup] at the end of every group of kGroup elements
+// and after element n-1.  A range that starts at 0 (from s = +0) adds its
+// first kExactHead terms serially.  Called by every thread of a
+// kChainThreads workgroup; returns the sum in every thread.
+__device__ __forceinline__ float exact_sum_range(float s, uint32_t pos, uint32_t end, uint32_t n,
+                                                 const float* __restrict__ x, float* prefix, ExactLds& L) {
   const uint32_t tid = threadIdx.x;
-  float s = 0.0f;
-  uint32_t pos = 0;
   int dense = 0;
-  {  // the serial head
-    const uint32_t hend = min(n, kExactHead);
+  if (pos == 0) {  // the serial head
+    const uint32_t hend = min(end, kExactHead);
     for (uint32_t i = tid; i < hend; i += blockDim.x) L.stage[i] = x[i];
     __syncthreads();
-    if (tid == 0) L.s = serial_add(0.0f, L.stage, 0, hend, n, prefix);
+    if (tid == 0) L.s = serial_add(s, L.stage, 0, hend, n, prefix);
     __syncthreads();
     s = L.s;
     pos = hend;
     __syncthreads();
   }
-  while (pos < n) {  // uniform over the workgroup
-    const uint32_t tend = min(n, pos + kExactTile);
+  while (pos < end) {  // uniform over the workgroup
+    const uint32_t tend = min(end, pos + kExactTile);
     const uint32_t j0 = pos + tid * kExactPer;
     float xv[kExactPer];
 #pragma unroll
     for (int e = 0; e < kExactPer; ++e) xv[e] = (j0 + e < tend) ? x[j0 + e] : 0.0f;
     if (dense >= kExactDense || !isfinite(s)) {
-      s = serial_rest(s, pos, n, x, prefix, L);
-      pos = n;
+      s = serial_rest(s, pos, end, n, x, prefix, L);
+      pos = end;
       continue;
     }
     if (s == 0.0f) {  // +0: the sum is the first non-zero term
@@ -499,6 +503,10 @@ __device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__
   return s;
 }
 
+__device__ __forceinline__ float exact_sum(uint32_t n, const float* __restrict__ x, float* prefix, ExactLds& L) {
+  return exact_sum_range(0.0f, 0, n, n, x, prefix, L);
+}
+
 // Whether any of w[0..n) is negative or not finite: the running sum c is
 // then not monotone, and resample selects serially (the flag goes to
 // sums[4]).  Called by every thread of the workgroup.
@@ -559,6 +567,401 @@ __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) 
       sums_of(f)[4] = irregular;
     }
   }
+}
+
+// ------------------------------------------------------- tiled exact sums
+// A single large filter's sums, spread over the chip instead of one CU.  The
+// terms are cut into tiles of kTiledTile, each of 16 sub-tiles of 64 (one
+// wavefront, one resample checkpoint group).  While a running sum s stays in
+// one binade of spacing u (s = k*u), a run of terms acts on k as a two-state
+// map (parity in -> increment, parity out) that depends on the terms and u
+// only (the map of exact_sum, composed over the run).  So:
+//   dsum  one workgroup per tile: the tile's terms summed in double, and
+//         their signs;
+//   map   one workgroup per tile: from the double sums of the earlier terms,
+//         guesses the binade s is in where the tile starts, and where each
+//         sub-tile starts; composes the tile's map under the tile's guess,
+//         and each sub-tile's maps under its guess and the binade above;
+//   walk  one workgroup per sum, from s = +0.  Wavefront 0 takes each tile
+//         whose map applies (s in the guessed binade, every term of the
+//         sign of s, the tile ending inside the binade) in O(1); any other
+//         tile sub-tile by sub-tile: by a sub-tile map when one applies,
+//         else as 64 float adds in order; tiles with terms of both signs go
+//         to exact_sum_range, the one-CU algorithm, run by the whole
+//         workgroup from the exact s;
+//   ckpt  for resample's running sum c: the checkpoints (c after every 64th
+//         weight) of the tiles the walk took whole, from the exact s it
+//         recorded at their start (the walk writes the others').
+// The guesses only decide which path a run of terms takes, never a bit of
+// the result: every sum is the sequential float sum exactly.
+constexpr uint32_t kTiledMin = 16384;  // particles from which a single filter's sums run tiled
+constexpr uint32_t kWalkTiles = 1024;  // tile records the walk holds in LDS at a time
+constexpr uint32_t kSubTiles = kTiledTile / 64;
+
+struct TileRec {
+  double dsum;          // dsum: the tile's terms summed in double
+  double asum;          // dsum: their magnitudes summed in double
+  uint32_t flags;       // dsum: kPos, kNeg, kIrregularTerm; map: kNoMap
+  uint32_t d0, d1, pp;  // map: the tile's map under binade e (pp = p0 | p1 << 1)
+  int32_t e;            // map: the binade exponent the map is for
+  uint32_t regular;     // walk: the tile was added whole from its map
+  float s_in;           // walk: the exact running sum before the tile
+  uint32_t pad[5];
+};
+static_assert(sizeof(TileRec) == 64, "16 floats per tile record");
+// A sub-tile's maps under its guessed binade e (a) and e + 1 (b).
+struct SubRec {
+  uint32_t d0a, d1a, d0b, d1b;
+  uint32_t pp;  // p0a | p1a << 1 | p0b << 2 | p1b << 3
+  int32_t e;
+  uint32_t flags;  // kNoMapA, kNoMapB, kPos, kNeg
+  uint32_t pad;
+};
+static_assert(sizeof(SubRec) == 32, "8 floats per sub-tile record");
+enum TileFlags : uint32_t { kNoMap = 1u, kPos = 2u, kNeg = 4u, kIrregularTerm = 8u, kNoMapA = 16u, kNoMapB = 32u };
+
+__device__ inline TileRec* tiled_recs(const FilterDev& f, uint32_t k) {
+  return reinterpret_cast<TileRec*>(f.scratch + sc_tiles(f.n)) + (size_t)k * tiled_tiles(f.n);
+}
+__device__ inline SubRec* tiled_subs(const FilterDev& f, uint32_t k) {
+  const size_t T = tiled_tiles(f.n);
+  return reinterpret_cast<SubRec*>(f.scratch + sc_tiles(f.n) + 16 * kTiledSums * T) + (size_t)k * T * kSubTiles;
+}
+
+// Sum k of a tiled launch: its terms, and where its result goes.
+__device__ inline const float* tiled_src(const FilterDev& f, int which, uint32_t k) {
+  if (which == kTailTotals || (which == kMeanPrefix && k < 4)) return f.scratch + (size_t)k * f.n;
+  return f.w;
+}
+__device__ inline bool tiled_has_prefix(int which, uint32_t k) {
+  return which == kPrefixOnly || (which == kMeanPrefix && k == 4);
+}
+__device__ inline float* tiled_dst(const FilterDev& f, int which, uint32_t k) {
+  if (which == kTailTotals) return sums_of(f) + k;
+  if (which == kWeightTotal) return sums_of(f) + 2;
+  if (which == kMeanPrefix && k < 4) return f.mean + k;
+  return sums_of(f) + 3;
+}
+
+__device__ inline int float_exponent(float a) { return (int)((__float_as_uint(a) >> 23) & 0xffu) - 127; }
+
+// The map of adding |x| to k*u (r = |x|/u exactly, a power-of-two scale):
+// t = r rounded to the nearest integer, ties making k + t even.  hard: |x| is
+// too large for the run to stay in the binade (or not finite).
+__device__ __forceinline__ ParityFn term_map(float x, double inv_u, bool& hard) {
+  const double r = fabs((double)x) * inv_u;
+  hard = !(r < 0x1p24);
+  ParityFn f{0u, 0u, 0u, 1u};
+  if (!hard) {
+    const double qd = floor(r), fr = r - qd;
+    const uint32_t q = (uint32_t)qd;
+    if (fr == 0.5) {
+      f.d0 = q + (q & 1u);
+      f.d1 = q + ((q + 1u) & 1u);
+      f.p0 = 0u;
+      f.p1 = 0u;
+    } else {
+      const uint32_t t = q + (fr > 0.5 ? 1u : 0u);
+      f.d0 = f.d1 = t;
+      f.p0 = t & 1u;
+      f.p1 = (t + 1u) & 1u;
+    }
+  }
+  return f;
+}
+
+// The composition of the 64 lanes' maps in lane order, in lane 63.
+__device__ __forceinline__ ParityFn wave_total_fn(ParityFn f) {
+  ParityFn inc = row_inclusive_fn(f);
+  inc = fn_then(dpp_fn<0x142, 0xa>(inc), inc);
+  inc = fn_then(dpp_fn<0x143, 0xc>(inc), inc);
+  return inc;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_sum_d(v);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kChainThreads / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+struct ScanLds {
+  uint32_t d0[kChainThreads / 64], d1[kChainThreads / 64], pp[kChainThreads / 64];
+};
+
+// One workgroup per (tile, sum), one term per thread.
+__global__ __launch_bounds__(kChainThreads) void tiled_dsum_kernel(Filters fs, int which) {
+  __shared__ double red[kChainThreads / 64];
+  const FilterDev& f = fs.one;
+  const uint32_t n = f.n, t = blockIdx.x, k = blockIdx.y;
+  const uint32_t i = t * kTiledTile + threadIdx.x;
+  const float xi = i < n ? tiled_src(f, which, k)[i] : 0.0f;
+  const double ds = block_sum_d((double)xi, red);
+  const double as = block_sum_d(fabs((double)xi), red);
+  const int pos = __syncthreads_or(xi > 0.0f ? 1 : 0);
+  const int neg = __syncthreads_or(xi < 0.0f ? 1 : 0);
+  const int irr = __syncthreads_or((i < n && (xi < 0.0f || !isfinite(xi))) ? 1 : 0);
+  if (threadIdx.x == 0) {
+    TileRec* R = tiled_recs(f, k) + t;
+    R->dsum = ds;
+    R->asum = as;
+    R->flags = (pos ? kPos : 0u) | (neg ? kNeg : 0u) | (irr ? kIrregularTerm : 0u);
+  }
+}
+
+__device__ inline bool normal_guess(double v, int& e) {
+  const float g = fabsf((float)v);
+  const bool ok = g >= 0x1p-126f && g <= 0x1p127f;  // normal (and finite)
+  e = ok ? float_exponent(g) : 0;
+  return ok;
+}
+
+__global__ __launch_bounds__(kChainThreads) void tiled_map_kernel(Filters fs, int which) {
+  __shared__ ScanLds L;
+  __shared__ double red[kChainThreads / 64];
+  const FilterDev& f = fs.one;
+  const uint32_t n = f.n, t = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  TileRec* recs = tiled_recs(f, k);
+  // the double sum of every term before the tile, then before each sub-tile
+  double before = 0.0;
+  for (uint32_t j = tid; j < t; j += kChainThreads) before += recs[j].dsum;
+  before = block_sum_d(before, red);
+  const uint32_t i = t * kTiledTile + tid;
+  const float xi = i < n ? tiled_src(f, which, k)[i] : 0.0f;
+  const double ws = wave_sum_d((double)xi);
+  if (lane == 0) red[wave] = ws;
+  __syncthreads();
+  double sub_before = before;
+  for (int w = 0; w < wave; ++w) sub_before += red[w];
+  __syncthreads();
+  // the tile's map under the tile's guess
+  int e;
+  const bool guess = normal_guess(before, e);
+  bool hard = !guess;
+  ParityFn m{0u, 0u, 0u, 1u};
+  if (guess) m = term_map(xi, ldexp(1.0, 23 - e), hard);
+  // the sub-tile's maps under its guess (a) and the binade above (b)
+  int es;
+  const bool sguess = normal_guess(sub_before, es);
+  bool hard_a = !sguess, hard_b = !sguess;
+  ParityFn ma{0u, 0u, 0u, 1u}, mb{0u, 0u, 0u, 1u};
+  if (sguess) {
+    ma = term_map(xi, ldexp(1.0, 23 - es), hard_a);
+    mb = term_map(xi, ldexp(1.0, 22 - es), hard_b);
+  }
+  const ParityFn ta = wave_total_fn(ma), tb = wave_total_fn(mb);
+  const bool sub_hard_a = __ballot(hard_a) != 0, sub_hard_b = __ballot(hard_b) != 0;
+  const bool sub_pos = __ballot(xi > 0.0f) != 0, sub_neg = __ballot(xi < 0.0f) != 0;
+  if (lane == 63) {
+    SubRec* S = tiled_subs(f, k) + (size_t)t * kSubTiles + wave;
+    S->d0a = ta.d0;
+    S->d1a = ta.d1;
+    S->d0b = tb.d0;
+    S->d1b = tb.d1;
+    S->pp = ta.p0 | (ta.p1 << 1) | (tb.p0 << 2) | (tb.p1 << 3);
+    S->e = es;
+    S->flags = (sub_hard_a ? kNoMapA : 0u) | (sub_hard_b ? kNoMapB : 0u) | (sub_pos ? kPos : 0u) |
+               (sub_neg ? kNeg : 0u);
+  }
+  const int any_hard = __syncthreads_or(hard ? 1 : 0);
+  const ParityFn ex = block_exclusive_fn(m, L);
+  if (tid == kChainThreads - 1) {
+    TileRec* R = recs + t;
+    const uint32_t fl = R->flags;
+    // terms adding up to 2^24 u or more leave the binade whatever k is (and
+    // the 32-bit increments could wrap); mixed signs have no map
+    const bool nomap = any_hard || !(R->asum * ldexp(1.0, 23 - e) < 0x1p24) || ((fl & kPos) && (fl & kNeg));
+    const ParityFn tot = fn_then(ex, m);
+    R->d0 = tot.d0;
+    R->d1 = tot.d1;
+    R->pp = tot.p0 | (tot.p1 << 1);
+    R->e = e;
+    R->flags = fl | (nomap ? kNoMap : 0u);
+  }
+}
+
+// s + (the run's increment of k under binade e): the new sum, or false when
+// the run does not stay in the binade of s (or s is not a normal number of
+// the terms' sign).
+__device__ __forceinline__ bool apply_map(float s, uint32_t flags, int e, uint32_t d0, uint32_t d1, float& out) {
+  if (!(fabsf(s) >= 0x1p-126f) || !isfinite(s) || float_exponent(fabsf(s)) != e) return false;
+  if ((s > 0.0f) ? (flags & kNeg) : (flags & kPos)) return false;
+  const uint32_t kin = (uint32_t)(fabs((double)s) * ldexp(1.0, 23 - e));  // in [2^23, 2^24)
+  const uint64_t kout = (uint64_t)kin + ((kin & 1u) ? d1 : d0);
+  if (kout >= (1ull << 24)) return false;
+  const float so = (float)((double)kout * ldexp(1.0, e - 23));
+  out = (s < 0.0f) ? -so : so;
+  return true;
+}
+
+struct WalkLds {
+  uint32_t d0[kWalkTiles], d1[kWalkTiles], fe[kWalkTiles];  // fe: flags | (e + 256) << 8
+  float s_in[kWalkTiles];
+  uint32_t regular[kWalkTiles];
+  float s;
+  uint32_t stop, run_end, irr;
+};
+
+// Tile t sub-tile by sub-tile (wavefront 0, every lane alike): s after it;
+// the checkpoint after every sub-tile goes to prefix (when given).
+__device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, const SubRec* subs, float* prefix) {
+  const int lane = threadIdx.x & 63;
+  SubRec r{};
+  if (lane < (int)kSubTiles) r = subs[(size_t)t * kSubTiles + lane];
+  for (uint32_t q = 0; q < kSubTiles; ++q) {
+    const uint32_t base = t * kTiledTile + q * 64;
+    if (base >= n) break;
+    const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)r.flags, q);
+    const int e = __builtin_amdgcn_readlane(r.e, q);
+    bool done = false;
+    if (s == 0.0f && !(flags & (kPos | kNeg))) {
+      done = true;  // +0 plus zeros stays +0
+    } else {
+      float so;
+      if (!(flags & kNoMapA) &&
+          apply_map(s, flags, e, (uint32_t)__builtin_amdgcn_readlane((int)r.d0a, q),
+                    (uint32_t)__builtin_amdgcn_readlane((int)r.d1a, q), so)) {
+        s = so;
+        done = true;
+      } else if (!(flags & kNoMapB) &&
+                 apply_map(s, flags, e + 1, (uint32_t)__builtin_amdgcn_readlane((int)r.d0b, q),
+                           (uint32_t)__builtin_amdgcn_readlane((int)r.d1b, q), so)) {
+        s = so;
+        done = true;
+      }
+    }
+    if (!done) {  // the reference's float adds, in order
+      const uint32_t len = min(64u, n - base);
+      const float xv = ((uint32_t)lane < len) ? x[base + lane] : 0.0f;
+      for (uint32_t j = 0; j < len; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), j));
+    }
+    if (prefix && lane == 0) prefix[base / kGroup] = s;
+  }
+  return s;
+}
+
+// One workgroup per sum walks the tiles in order.
+__global__ __launch_bounds__(kChainThreads) void tiled_walk_kernel(Filters fs, int which) {
+  __shared__ ExactLds L;
+  __shared__ WalkLds W;
+  const FilterDev& f = fs.one;
+  const uint32_t n = f.n, k = blockIdx.y, tid = threadIdx.x;
+  const float* x = tiled_src(f, which, k);
+  float* prefix = tiled_has_prefix(which, k) ? f.scratch + sc_c32(n) : nullptr;
+  const uint32_t T = (uint32_t)tiled_tiles(n);
+  TileRec* recs = tiled_recs(f, k);
+  const SubRec* subs = tiled_subs(f, k);
+  float s = 0.0f;
+  if (tid == 0) W.irr = 0u;
+  for (uint32_t c0 = 0; c0 < T; c0 += kWalkTiles) {  // a chunk of tile records in LDS
+    const uint32_t c1 = min(T, c0 + kWalkTiles);
+    for (uint32_t j = c0 + tid; j < c1; j += kChainThreads) {
+      const TileRec& r = recs[j];
+      W.d0[j - c0] = r.d0;
+      W.d1[j - c0] = r.d1;
+      W.fe[j - c0] = r.flags | ((uint32_t)(r.e + 256) << 8);
+      W.regular[j - c0] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t t = c0; t < c1;) {  // uniform over the workgroup
+      if (tid < 64) {
+        // wavefront 0 walks: whole tiles by their maps, others sub-tile by
+        // sub-tile, until a tile with terms of both signs
+        uint32_t irr = W.irr;
+        for (; t < c1; ++t) {
+          const uint32_t j = t - c0, fl = W.fe[j] & 0xffu;
+          const int e = (int)(W.fe[j] >> 8) - 256;
+          if (tid == 0) W.s_in[j] = s;
+          irr |= fl & kIrregularTerm;
+          if (s == 0.0f && !(fl & (kPos | kNeg))) {  // +0 plus zeros: checkpoints of +0
+            if (tid == 0) W.regular[j] = 1u;
+            continue;
+          }
+          float so;
+          if (!(fl & kNoMap) && apply_map(s, fl, e, W.d0[j], W.d1[j], so)) {
+            s = so;
+            if (tid == 0) W.regular[j] = 1u;
+            continue;
+          }
+          if ((fl & kPos) && (fl & kNeg)) break;
+          s = walk_subtiles(s, t, n, x, subs, prefix);
+        }
+        // the tiles from the one that stopped it while their terms have both signs
+        uint32_t end = t < c1 ? t + 1 : c1;
+        while (end < c1 && (W.fe[end - c0] & kPos) && (W.fe[end - c0] & kNeg)) {
+          irr |= W.fe[end - c0] & kIrregularTerm;
+          ++end;
+        }
+        if (tid == 0) {
+          W.irr = irr;
+          W.s = s;
+          W.stop = t;
+          W.run_end = end;
+        }
+      }
+      __syncthreads();
+      s = W.s;
+      const uint32_t stop = W.stop, end = W.run_end;
+      __syncthreads();
+      if (stop < c1) s = exact_sum_range(s, stop * kTiledTile, min(n, end * kTiledTile), n, x, prefix, L);
+      t = end;
+    }
+    for (uint32_t j = c0 + tid; j < c1; j += kChainThreads) {
+      recs[j].s_in = W.s_in[j - c0];
+      recs[j].regular = W.regular[j - c0];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    *tiled_dst(f, which, k) = s;
+    if (prefix) sums_of(f)[4] = W.irr ? 1.0f : 0.0f;
+  }
+}
+
+// Checkpoints of resample's running sum in the tiles the walk added whole
+// from their maps (the walk, or exact_sum_range, wrote the others').
+__global__ __launch_bounds__(kChainThreads) void tiled_ckpt_kernel(Filters fs, int which) {
+  __shared__ ScanLds L;
+  const FilterDev& f = fs.one;
+  const uint32_t n = f.n, t = blockIdx.x, tid = threadIdx.x;
+  const uint32_t k = (which == kMeanPrefix) ? 4u : 0u;
+  const TileRec* R = tiled_recs(f, k) + t;
+  if (!R->regular) return;  // uniform over the workgroup
+  float* prefix = f.scratch + sc_c32(n);
+  const uint32_t i = t * kTiledTile + tid;
+  const float xi = i < n ? tiled_src(f, which, k)[i] : 0.0f;
+  const int e = R->e;
+  bool hard;
+  const ParityFn m = term_map(xi, ldexp(1.0, 23 - e), hard);
+  const ParityFn ex = block_exclusive_fn(m, L);
+  const float s_in = R->s_in;
+  const uint32_t kin = (uint32_t)(fabs((double)s_in) * ldexp(1.0, 23 - e));
+  const uint32_t p = kin & 1u;
+  const uint32_t kb = kin + (p ? ex.d1 : ex.d0);
+  const uint32_t pb = p ? ex.p1 : ex.p0;
+  const uint32_t ka = kb + (pb ? m.d1 : m.d0);
+  const double v = (double)ka * ldexp(1.0, e - 23);
+  if (i < n && is_checkpoint(i, n)) prefix[i / kGroup] = (float)(s_in < 0.0f ? -v : v);
+}
+
+inline bool tiled_filter(const Filters& fs) { return !fs.arr && fs.one.n >= kTiledMin; }
+
+void launch_tiled(const Filters& fs, int which, uint32_t nsums, hipStream_t s) {
+  const uint32_t T = (uint32_t)tiled_tiles(fs.one.n);
+  tiled_dsum_kernel<<<dim3(T, nsums), kChainThreads, 0, s>>>(fs, which);
+  tiled_map_kernel<<<dim3(T, nsums), kChainThreads, 0, s>>>(fs, which);
+  tiled_walk_kernel<<<dim3(1, nsums), kChainThreads, 0, s>>>(fs, which);
+  if (which == kPrefixOnly || which == kMeanPrefix) tiled_ckpt_kernel<<<T, kChainThreads, 0, s>>>(fs, which);
 }
 
 // ----------------------------------------------------------------- predict
@@ -844,6 +1247,15 @@ void launch_init(const Filters& fs, hipStream_t s) {
 
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
+  if (tiled_filter(fs)) {
+    tail_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(g, fs);
+    launch_tiled(fs, kTailTotals, 2, s);
+    tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+    launch_tiled(fs, kWeightTotal, 1, s);
+    tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
+    launch_tiled(fs, kMeanPrefix, 5, s);
+    return;
+  }
   tail_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(g, fs);
   exact_chain_kernel<<<dim3(fs.count, 2), kChainThreads, 0, s>>>(fs, kTailTotals);
   tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
@@ -856,7 +1268,12 @@ void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
 void launch_resample(const Filters& fs, int mode, bool have_prefix, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
   if (mode == A3D_RESAMPLE_REFERENCE) {
-    if (!have_prefix) exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kPrefixOnly);
+    if (!have_prefix) {
+      if (tiled_filter(fs))
+        launch_tiled(fs, kPrefixOnly, 1, s);
+      else
+        exact_chain_kernel<<<fs.count, kChainThreads, 0, s>>>(fs, kPrefixOnly);
+    }
   } else {
     prefix64_kernel<<<fs.count, 1024, 0, s>>>(fs);
   }

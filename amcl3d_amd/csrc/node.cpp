@@ -174,8 +174,8 @@ struct a3d_node {
   std::vector<float> ranges;  // {r, ax, ay, az} per measurement since the last update
   std::vector<float> z4;
   // device draws of a predict step: 4 normals per particle, the step's
-  // uniform, and the (seed, step) key
-  hipStream_t stream = nullptr;
+  // uniform, and the (seed, step) key (made on the filter's stream)
+  uint64_t key[2] = {0, 0};
   float* d_z4 = nullptr;
   float* d_u01 = nullptr;
   uint64_t* d_key = nullptr;
@@ -244,8 +244,7 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
     delete nd;
     return rc;
   }
-  if (hipStreamCreateWithFlags(&nd->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&nd->d_z4, 16 * (size_t)params->num_particles) != hipSuccess ||
+  if (hipMalloc(&nd->d_z4, 16 * (size_t)params->num_particles) != hipSuccess ||
       hipMalloc(&nd->d_u01, 4) != hipSuccess || hipMalloc(&nd->d_key, 16) != hipSuccess ||
       !(nd->vws = voxel_workspace_new())) {
     a3d_node_destroy(nd);
@@ -257,8 +256,7 @@ int a3d_node_create(const a3d_node_params* params, const a3d_grid* grid, uint64_
 
 int a3d_node_destroy(a3d_node* nd) {
   if (!nd) return A3D_OK;
-  if (nd->pf) a3d_pf_destroy(nd->pf);
-  if (nd->stream) (void)hipStreamSynchronize(nd->stream);
+  if (nd->pf) a3d_pf_destroy(nd->pf);  // waits for the filter's stream
   (void)hipFree(nd->d_z4);
   (void)hipFree(nd->d_u01);
   (void)hipFree(nd->d_key);
@@ -266,7 +264,6 @@ int a3d_node_destroy(a3d_node* nd) {
   (void)hipFree(nd->d_v);
   (void)hipFree(nd->d_rng);
   voxel_workspace_free(nd->vws);
-  if (nd->stream) (void)hipStreamDestroy(nd->stream);
   delete nd;
   return A3D_OK;
 }
@@ -405,16 +402,17 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   if (rc) return rc;
   if (cur > (size_t)nd->p.num_particles) return fail(A3D_ERR_STATE, "a3d_node_point_cloud: filter outgrew the node");
   // this step's draws, as a3d_session_draws gives them, made on the device
-  const uint64_t key[2] = {nd->seed, nd->step};
-  float u01 = 0.f;
+  // on the filter's stream, so predict and resample read them in order and
+  // the host never waits for them
+  const hipStream_t ps = pf_stream(nd->pf);
+  nd->key[0] = nd->seed;
+  nd->key[1] = nd->step;
   FilterDev f{};
   f.n = (uint32_t)cur;
   f.z4 = nd->d_z4;
-  A3D_HIP(hipMemcpyAsync(nd->d_key, key, 16, hipMemcpyHostToDevice, nd->stream));
-  launch_draws(single(f), nd->d_key, nd->d_key + 1, nd->d_u01, nd->stream);
+  A3D_HIP(hipMemcpyAsync(nd->d_key, nd->key, 16, hipMemcpyHostToDevice, ps));
+  launch_draws(single(f), nd->d_key, nd->d_key + 1, nd->d_u01, ps);
   A3D_HIP(hipGetLastError());
-  A3D_HIP(hipMemcpyAsync(&u01, nd->d_u01, 4, hipMemcpyDeviceToHost, nd->stream));
-  A3D_HIP(hipStreamSynchronize(nd->stream));
   ++nd->step;
   const double mods[4] = {nd->p.odom_x_mod, nd->p.odom_y_mod, nd->p.odom_z_mod, nd->p.odom_a_mod};
   rc = a3d_pf_predict_dev(nd->pf, mods, nd->deltas, nd->d_z4);
@@ -428,7 +426,9 @@ int a3d_node_point_cloud(a3d_node* nd, int64_t now_ns, const void* points, size_
   nd->lastupdatebase_2_odom = nd->base_2_odom;
   if (++nd->n_updates > nd->p.resample_interval) {
     nd->n_updates = 0;
-    rc = a3d_pf_resample(nd->pf, u01, A3D_RESAMPLE_REFERENCE, nullptr);
+    // enqueued behind the update; the next callback's work waits for it on
+    // the same stream
+    rc = pf_resample_device_u01(nd->pf, nd->d_u01);
     if (rc) return rc;
   }
   *updated = 1;

@@ -43,12 +43,26 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
 }
 
-// Work-queue kernels split the (filter-major) item list into one contiguous
-// range per XCD, each with its own counter: workgroup b starts on range b mod
-// 8, the one of its own XCD, so an L2 serves the particles of few filters at a
-// time, and moves on to the other ranges (in turn) when its own is empty.
+// Work-queue kernels deal the (filter-major) item list to the XCDs, each
+// share with its own counter: workgroup b starts on the share of its own XCD
+// (b mod 8) and moves on to the other shares (in turn) when its own is empty.
+// deal == 0: XCD x takes one contiguous range, so its L2 serves the particles
+// of few filters at a time.  deal == k > 0: XCD x takes runs of k consecutive
+// items, every 8th run, so all XCDs advance through the same part of the list
+// together and the chip works on few filters at a time (whose cells then stay
+// in the memory-side cache).
 constexpr uint32_t kXcds = 8;
 __device__ inline uint32_t xcd_range(uint32_t n, uint32_t x) { return (uint32_t)((uint64_t)n * x / kXcds); }
+__device__ inline uint32_t deal_len(uint32_t n, uint32_t x, uint32_t k) {
+  if (k == 0) return xcd_range(n, x + 1) - xcd_range(n, x);
+  const uint32_t rem = n % (kXcds * k), lo = x * k;
+  return (n / (kXcds * k)) * k + (rem > lo ? min(k, rem - lo) : 0u);
+}
+// The t-th item (t < deal_len) of XCD x's share.
+__device__ inline uint32_t deal_item(uint32_t n, uint32_t x, uint32_t k, uint32_t t) {
+  if (k == 0) return xcd_range(n, x) + t;
+  return ((t / k) * kXcds + x) * k + t % k;
+}
 
 constexpr int kTS = 68;        // tile row stride in floats (64 points + pad, 16-byte rows)
 constexpr int kWaves = 4;      // wavefronts per workgroup
@@ -466,11 +480,11 @@ __device__ __forceinline__ void lanes_chunk(const GridDev& g, const ProbSrc<IK>&
     // then this group's loads into the registers they free; the scheduling
     // barrier keeps the compiler from hoisting the adds (and their wait on
     // the loads) above the cell arithmetic
-    float v[U];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = src(idx[u]);
+    for (int u = 0; u < U; ++u) accumulate(pend[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) accumulate(v[u]);
+    for (int u = 0; u < U; ++u) pend[u] = src(idx[u]);
   }
 }
 
@@ -690,7 +704,7 @@ template <int Q, int IK, bool CLEAN>
 __global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs,
                                                                     const WorkItem* items, uint32_t nitems,
                                                                     uint32_t* counter, int persistent,
-                                                                    uint32_t chunk) {
+                                                                    uint32_t chunk, uint32_t deal) {
   __shared__ float4 cloud[2 * kMaxChunk];
   __shared__ uint32_t taken;
   if (!persistent) {
@@ -700,14 +714,14 @@ __global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, c
     return;
   }
   for (uint32_t r = 0; r < kXcds; ++r) {
-    const uint32_t x = (blockIdx.x + r) % kXcds, beg = xcd_range(nitems, x), len = xcd_range(nitems, x + 1) - beg;
+    const uint32_t x = (blockIdx.x + r) % kXcds, len = deal_len(nitems, x, deal);
     for (;;) {
       if (threadIdx.x == 0) taken = atomicAdd(counter + x, 1u);
       __syncthreads();
       const uint32_t t = taken;
       __syncthreads();  // every wavefront has read it before the next take
       if (t >= len) break;  // uniform over the workgroup
-      const WorkItem it = items[beg + t];
+      const WorkItem it = items[deal_item(nitems, x, deal, t)];
       lanes_item<Q, IK, CLEAN>(g, fs[it.filter], it, cloud, chunk);
     }
   }
@@ -754,7 +768,7 @@ __device__ __forceinline__ void block_item(const GridDev& g, const FilterDev& f,
 template <int PB, bool IDX64>
 __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const FilterDev* fs,
                                                                 const WorkItem* items, uint32_t nitems,
-                                                                uint32_t* counter, int persistent) {
+                                                                uint32_t* counter, int persistent, uint32_t deal) {
   __shared__ float4 tiles4[kWaves * PB * kTS / 4];
   float* tile = reinterpret_cast<float*>(tiles4) + (threadIdx.x >> 6) * PB * kTS;
   const int lane = lane_id();
@@ -765,13 +779,14 @@ __global__ __launch_bounds__(kThreads) void weight_fleet_kernel(GridDev g, const
     return;
   }
   for (uint32_t r = 0; r < kXcds; ++r) {
-    const uint32_t x = (blockIdx.x + r) % kXcds, beg = xcd_range(nitems, x), len = xcd_range(nitems, x + 1) - beg;
+    const uint32_t x = (blockIdx.x + r) % kXcds, len = deal_len(nitems, x, deal);
     for (;;) {
       uint32_t t = 0;
       if (lane == 0) t = atomicAdd(counter + x, 1u);
       t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
       if (t >= len) break;
-      block_item<PB, IDX64>(g, fs[items[beg + t].filter], items[beg + t], tile);
+      const WorkItem& it = items[deal_item(nitems, x, deal, t)];
+      block_item<PB, IDX64>(g, fs[it.filter], it, tile);
     }
   }
 }
@@ -823,10 +838,11 @@ void launch_fleet_pb(const GridDev& g, const FilterDev* fs, const WorkItem* item
     blocks = (nitems + kWaves - 1) / kWaves;
   }
   if (blocks == 0) return;
+  const uint32_t deal = (uint32_t)current_deal();
   if (g.idx64)
-    weight_fleet_kernel<PB, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+    weight_fleet_kernel<PB, true><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent, deal);
   else
-    weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent);
+    weight_fleet_kernel<PB, false><<<blocks, kThreads, 0, s>>>(g, fs, items, nitems, counter, persistent, deal);
 }
 
 // The index kind a launch reads its grids with: the widest its maps need.
@@ -911,8 +927,9 @@ void lanes_fleet_q(const GridDev& g, const FilterDev* fs, const WorkItem* items,
   }
   if (blocks == 0) return;
   const dim3 b(blocks), t(kBlockThreads);
+  const uint32_t deal = (uint32_t)current_deal();
 #define A3D_LAUNCH b, t, 0, s
-  A3D_LANES_VARIANT(v, lanes_fleet_kernel, Q, A3D_LAUNCH, g, fs, items, nitems, counter, persistent, chunk)
+  A3D_LANES_VARIANT(v, lanes_fleet_kernel, Q, A3D_LAUNCH, g, fs, items, nitems, counter, persistent, chunk, deal)
 #undef A3D_LAUNCH
 }
 

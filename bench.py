@@ -32,17 +32,16 @@ sys.path.insert(0, ROOT)
 METRIC = "particle·point evals/s; p50/p99 update latency, served vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVAL = 4  # one FP32 prob gather
-# HBM bytes per cloud-weight launch of this workload at its default shape,
-# averaged over the 100 timed steps: rocprofv3 --pmc FETCH_SIZE and --pmc
-# WRITE_SIZE over `python -m amcl3d_amd.profile_run --updates 103 --shape
-# 0,16,1`, each scaled by the counter's reading for a 1 GiB device copy
-# (2.045e9 read + 4.7e6 written).  Counters need their own profiler runs, so
-# the figure is measured apart from this run.
-TRAFFIC_BYTES = 2.05e9
-# Vector instructions a wavefront issues per 64 evaluations of this kernel
-# (rocprofv3 --pmc SQ_INSTS_VALU over profile_run, steady state), and the
-# chip's issue rate: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64 instruction.
-VALU_PER_64_EVALS = 34.9
+# Per-update counters of the cloud-weight kernel on this workload (every step
+# k of a seed's workload is the same whatever the run's length): HBM bytes
+# read and written (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, separate runs,
+# each scaled by its reading for a 1 GiB device copy) and vector instructions
+# issued (SQ_INSTS_VALU).  Counters need profiler runs of their own, so the
+# table is measured apart from this run; the bench averages it over exactly
+# the updates it times.
+PROFILE_TABLE = os.path.join(ROOT, "profiles", "cloud_weight_large_filter_per_update.md")
+# The chip's vector issue rate: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64
+# instruction.
 ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
@@ -68,8 +67,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--shape", default="auto",
-                    help="cloud-weight kernel shape 'MAPPING,PPW,ORDER' (e.g. 0,16,1) or 'auto' to time every shape")
+    ap.add_argument("--shape", default="profiled",
+                    help="cloud-weight kernel shape 'MAPPING,PPW,ORDER[,TILE]' (e.g. 0,16,1,64), 'auto' to time "
+                         "every shape and keep the fastest, or 'profiled' (default): the shape of the profile table "
+                         "(the autotuner's choice when it was profiled), else 'auto'")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU. Filters and sessions are independent, so several GPUs would each "
@@ -133,6 +134,7 @@ def main():
     pf.set_particles(start_state)
     pf.timing(reset=True)
     inmap_evals = 0
+    evals_per_update = []
     bm = wl["bm"]
     for k in range(Wm, Wm + K):
         x, y, z, r, dz = dbuf[k]
@@ -141,6 +143,7 @@ def main():
         p = pf.particles()
         inside = np.all((p[:, :3].astype(np.float64) >= bm.min) & (p[:, :3].astype(np.float64) < bm.max), 1)
         inmap_evals += int(inside.sum()) * args.points
+        evals_per_update.append(int(inside.sum()) * args.points)
         pf.update_dev(grid, x.ptr, y.ptr, z.ptr, args.points, r.ptr, args.beacons, alpha, sigma)
         pf.resample_dev(wl["u01"][k])
     weight_ms, launches, _ = pf.timing(reset=True)
@@ -151,6 +154,8 @@ def main():
 
     value = inmap_evals / elapsed
     evals_per_launch = inmap_evals / K
+    counters = profile_counters(PROFILE_TABLE, Wm, K, evals_per_update, tuning["chosen"], seed=args.seed,
+                                n=args.particles, m=args.points)
     achieved = evals_per_launch * BYTES_PER_EVAL / (weight_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -184,17 +189,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": TRAFFIC_BYTES,
-            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (separate runs, same "
-                            "workload and shape, mean of the 100 timed steps), 0.26x the algorithmic bytes: "
-                            "the L2 serves the gathers of nearby poses",
-            "issue": {"valu_instr_per_64_evals": VALU_PER_64_EVALS,
-                      "achieved_wave_instr_per_s": evals_per_launch / 64 * VALU_PER_64_EVALS / (weight_ms * 1e-3),
-                      "peak_wave_instr_per_s": ISSUE_PEAK,
-                      "frac": evals_per_launch / 64 * VALU_PER_64_EVALS / (weight_ms * 1e-3) / ISSUE_PEAK,
-                      "note": "the kernel is bound by vector instruction issue, not HBM: per evaluation the "
-                              "reference's FP64 offset add, float rounding, bounds and cell arithmetic per axis"},
+            "traffic": counters.get("traffic"),
+            "traffic_note": counters["note"],
+            "issue": issue_block(counters, evals_per_launch, weight_ms),
             "kernel_ms_avg": weight_ms,
+            "kernel_ms_profiled": counters.get("kernel_ms"),
             "kernel_launches": launches,
             "algorithmic_bytes_per_launch": evals_per_launch * BYTES_PER_EVAL,
         },
@@ -211,10 +210,75 @@ def main():
 
             out["extras"] = benchlib.extras(amcl3d_amd, W, plain_value=value,
                                             building=(W.storey_view(wl["bm"], 1.5, 3.0), grid, 1.5),
-                                            building_map=wl["bm"], wl=wl, log=log)
+                                            building_map=wl["bm"], wl=wl, log=log,
+                                            valu_per_64_evals=counters.get("valu_per_64_evals"))
         except Exception as e:  # extras never hide the main line
             out["extras"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
+
+
+def profile_counters(path, warmup, steps, evals_per_update, shape, seed, n, m):
+    """Means over updates warmup+1 .. warmup+steps (1-based, as profiled) of the
+    profiled per-update counters, when the table is for this workload and
+    kernel shape."""
+    out = {"note": "no profile table"}
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return out
+    meta, rows = {}, {}
+    for line in lines:
+        if line.startswith("- ") and ": " in line:
+            k, v = line[2:].split(": ", 1)
+            meta[k.strip()] = v.strip().strip("`")
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0].isdigit():
+            rows[int(cells[0])] = [float(c) for c in cells[1:]]
+    want = {"shape": shape, "seed": str(seed), "particles": str(n), "points": str(m)}
+    if any(meta.get(k) != v for k, v in want.items()):
+        out["note"] = "the profile table (%s) is for %s, this run is %s" % (
+            os.path.relpath(path, ROOT), {k: meta.get(k) for k in want}, want)
+        return out
+    upd = list(range(warmup + 1, warmup + steps + 1))
+    if not all(u in rows for u in upd):
+        out["note"] = "the profile table covers updates 1..%d, this run times %d..%d" % (max(rows), upd[0], upd[-1])
+        return out
+    fetch = [rows[u][0] for u in upd]
+    write = [rows[u][1] for u in upd]
+    valu = [rows[u][2] for u in upd]
+    out["traffic"] = float(sum(fetch) + sum(write)) / len(upd)
+    out["read"] = float(sum(fetch)) / len(upd)
+    out["written"] = float(sum(write)) / len(upd)
+    out["valu_per_64_evals"] = float(sum(valu)) / (sum(evals_per_update) / 64.0)
+    if all(len(rows[u]) > 3 for u in upd):
+        out["kernel_ms"] = float(sum(rows[u][3] for u in upd)) / len(upd) / 1e3
+    out["note"] = ("HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch (separate runs of the same "
+                   "workload and shape, each scaled by a 1 GiB copy's reading), mean over updates %d..%d, the ones "
+                   "timed here (%s)" % (upd[0], upd[-1], os.path.relpath(path, ROOT)))
+    return out
+
+
+def profiled_shape(path):
+    """The kernel shape ('MAPPING,PPW,ORDER,TILE') the profile table names."""
+    try:
+        for line in open(path):
+            if line.startswith("- shape_args: "):
+                return line.split(": ", 1)[1].strip().strip("`")
+    except OSError:
+        pass
+    return None
+
+
+def issue_block(counters, evals_per_launch, weight_ms):
+    if "valu_per_64_evals" not in counters:
+        return None
+    v = counters["valu_per_64_evals"]
+    rate = evals_per_launch / 64 * v / (weight_ms * 1e-3)
+    return {"valu_instr_per_64_evals": v, "achieved_wave_instr_per_s": rate, "peak_wave_instr_per_s": ISSUE_PEAK,
+            "frac": rate / ISSUE_PEAK,
+            "note": "SQ_INSTS_VALU per dispatch from the same profile table, over the same updates; the kernel is "
+                    "bound by vector instruction issue, not HBM: per evaluation the reference's FP64 offset add, "
+                    "float rounding, bounds and cell arithmetic per axis"}
 
 
 def choose_shape(amcl3d_amd, pf, grid, bufs, args, alpha, sigma):
@@ -222,10 +286,13 @@ def choose_shape(amcl3d_amd, pf, grid, bufs, args, alpha, sigma):
     shape on the update of the first timed step (all shapes give identical bits)."""
     from amcl3d_amd import autotune
 
-    if args.shape != "auto":
-        sh = tuple(int(v) for v in args.shape.split(","))
+    spec = args.shape
+    if spec == "profiled":
+        spec = profiled_shape(PROFILE_TABLE) or "auto"
+    if spec != "auto":
+        sh = tuple(int(v) for v in spec.split(","))
         autotune.set_shape(sh)
-        return {"chosen": autotune.shape_name(sh)}
+        return {"chosen": autotune.shape_name(sh), "from": args.shape}
     x, y, z, r, _ = bufs
 
     def run():
@@ -239,9 +306,14 @@ def choose_shape(amcl3d_amd, pf, grid, bufs, args, alpha, sigma):
 
 
 def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):
-    """The oracle (the CPU restatement of the reference) on one host core, on a
-    bounded sample of the first timed step: its predict and update for the
-    first S particles with the full cloud; S is sized to about args.cpu_seconds."""
+    """The oracle (the CPU restatement of the reference) on one host core.
+
+    value: a bounded sample of the first timed step of this workload, its
+    predict and update for the first S particles with the full cloud; S is
+    sized to about args.cpu_seconds.  legs: the room configuration's full
+    predict -> update -> resample step (1,000 particles, 2,000 points, 3
+    beacons; median of 7) and the host computeGrid (exact nearest neighbour)
+    of that 200 x 200 x 60 map, both single-threaded like the reference."""
     import oracle
 
     from amcl3d_amd import workloads as W
@@ -261,20 +333,68 @@ def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):
     s = int(min(args.particles, max(100, 100 * args.cpu_seconds / max(t, 1e-6))))
     t = run(s)
     del g
+    legs = {}
+    # the room configuration: map build, then one full step
+    c = W.room_case(1)
+    bm = c["map"]
+    pts = bm.points()
+    t0 = time.perf_counter()
+    _, prob = oracle.compute_grid(pts, bm.min, bm.max, bm.res, c["sensor_dev"], nthreads=1)
+    tg = time.perf_counter() - t0
+    cells = int(np.prod(bm.size))
+    legs["room_compute_grid"] = {"cells": cells, "occupied_points": int(pts.shape[0]), "seconds": tg,
+                                 "cells_per_s": cells / tg, "threads": 1}
+    og = oracle.GridMeta(bm.min, bm.max, bm.res, prob)
+    st, parts = c["step"], c["particles"]
+    full, upd = [], []
+    for _ in range(7):
+        t0 = time.perf_counter()
+        op = oracle.pf_predict(parts, st.mods, st.deltas, st.z4)
+        t1 = time.perf_counter()
+        op, _ = oracle.pf_update(op, og, st.cloud, st.ranges, st.alpha, st.sigma, nthreads=1)
+        t2 = time.perf_counter()
+        oracle.pf_resample(op, st.u01)
+        full.append(time.perf_counter() - t0)
+        upd.append(t2 - t1)
+    inside = np.all((parts[:, :3].astype(np.float64) >= bm.min) & (parts[:, :3].astype(np.float64) < bm.max), 1)
+    ev = int(inside.sum()) * st.cloud.shape[0]
+    legs["room_full_step"] = {"particles": int(parts.shape[0]), "points": int(st.cloud.shape[0]),
+                              "beacons": int(st.ranges.shape[0]), "full_step_ms_median": 1e3 * float(np.median(full)),
+                              "update_ms_median": 1e3 * float(np.median(upd)),
+                              "update_evals_per_s": ev / float(np.median(upd)), "runs": len(full), "threads": 1}
     return {"value": s * args.points / t, "unit": "evals/s", "cores": 1, "kind": "port",
             "sample": "oracle ParticleFilter::update of the first %d of %d particles x %d points, one step, "
                       "single thread, %.1f s" % (s, args.particles, args.points, t),
-            "host_cpu": _cpu_model()}
+            "host": host_info(), "legs": legs}
 
 
-def _cpu_model():
+def host_info():
+    """The host CPU as lscpu names it, and how many cores this process may use."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key in ("Model name", "BIOS Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                       "CPU(s)", "CPU max MHz"):
+                info["lscpu " + key] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+                info["cpuinfo model name"] = line.split(":", 1)[1].strip()
+                break
     except OSError:
         pass
-    return "unknown"
+    return info
 
 
 if __name__ == "__main__":

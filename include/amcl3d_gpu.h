@@ -229,6 +229,14 @@ int a3d_get_tile(int* particles_per_wave);
  * always stage 64 (one point per lane). */
 int a3d_set_cloud_tile(int points);
 int a3d_get_cloud_tile(int* points);
+/* How persistent launches (fleets, served batches) deal their work list of
+ * particle blocks to the chip's 8 XCDs (accelerator dies, each with its own
+ * L2): 0 (default), one contiguous range per XCD, so an L2 serves few filters
+ * at a time; run > 0, runs of `run` consecutive blocks dealt round-robin, so
+ * the whole chip works on few filters at a time and their cells stay in the
+ * memory-side cache.  Results are identical either way. */
+int a3d_set_deal(int run);
+int a3d_get_deal(int* run);
 int a3d_set_mapping(int mapping);
 int a3d_get_mapping(int* mapping);
 /* Order in which the cloud-weight kernel visits a filter's particles (the

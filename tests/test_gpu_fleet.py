@@ -56,14 +56,15 @@ def test_fleet_matches_single_filters(gpu):
                 _lib.f64(roll), _lib.f64(pitch)]
         expect = [oracle.pf_update(parts[f], og, clouds[f], rngs[f], alpha[f], sigma[f], roll[f], pitch[f])
                   for f in range(F)]
-        m0, c0 = gpu.get_mapping(), gpu.get_cloud_tile()
+        m0, c0, d0 = gpu.get_mapping(), gpu.get_cloud_tile(), gpu.get_deal()
         try:
-            for mapping, tile in ((gpu.MAP_PARTICLE_LANES, 64), (gpu.MAP_PARTICLE_LANES, 256),
-                                  (gpu.MAP_PARTICLE_LANES, 32), (gpu.MAP_POINT_LANES, 64)):
+            for mapping, tile, deal in ((gpu.MAP_PARTICLE_LANES, 64, 0), (gpu.MAP_PARTICLE_LANES, 256, 4),
+                                        (gpu.MAP_PARTICLE_LANES, 32, 1), (gpu.MAP_POINT_LANES, 64, 3)):
                 gpu.set_mapping(mapping)
                 gpu.set_cloud_tile(tile)
+                gpu.set_deal(deal)
                 for persistent in (1, 0, -1):
-                    print("fleet mapping", mapping, "tile", tile, "persistent", persistent, flush=True)
+                    print("fleet mapping", mapping, "tile", tile, "deal", deal, "persistent", persistent, flush=True)
                     _lib.call("a3d_fleet_set_particles", fl, _lib.f32(P))
                     _lib.call("a3d_fleet_stage_update", fl, *args)
                     _lib.call("a3d_fleet_update_staged", fl, grid.handle, persistent)
@@ -76,13 +77,14 @@ def test_fleet_matches_single_filters(gpu):
                     for f in range(F):
                         op, om = expect[f]
                         n = int(counts[f])
-                        key = (mapping, tile, persistent, f)
+                        key = (mapping, tile, deal, persistent, f)
                         assert np.array_equal(out[off:off + n].view(np.int32), op.view(np.int32)), key
                         assert np.array_equal(means[f].view(np.int32), om.view(np.int32)), key
                         off += n
         finally:
             gpu.set_mapping(m0)
             gpu.set_cloud_tile(c0)
+            gpu.set_deal(d0)
         _lib.call("a3d_fleet_resample", fl, _lib.ptr(u01), 0)
         res = np.empty_like(P)
         _lib.call("a3d_fleet_get_particles", fl, _lib.ptr(res))

@@ -4,6 +4,8 @@ Every oracle function is checked against something that does not restate it:
 the reference's own known-answer values and fixtures (amcl3d/tests/*), closed
 forms, brute force on tiny inputs, and an FP64 evaluation.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -384,3 +386,50 @@ def test_voxel_grid_against_brute_force():
     out1 = oracle.voxel_grid(pts, leaf, order_mode=1)
     ulps = np.abs(out.view(np.int32).astype(np.int64) - out1.view(np.int32))
     assert ulps.max() <= 8
+
+
+def test_libm_choice_effect_on_room_steps():
+    """Which libm the reference's float calls resolve to (expf for prob,
+    PointCloudTools.cpp:137; sin/cos of the yaw, Grid3d.cpp:139-144 and
+    ParticleFilter.cpp:110-111) is fixed by its GCC 5 / glibc platform, not
+    by its source.  Measured here: how often glibc's expf and sinf/cosf differ
+    from the double routine rounded once to float (what the device computes),
+    and what the expf difference does to a full step on the room map (five
+    seeds): weights that change and resample indices that change."""
+    libm = ctypes.CDLL("libm.so.6")
+    for name in ("expf", "sinf", "cosf"):
+        getattr(libm, name).restype = ctypes.c_float
+        getattr(libm, name).argtypes = [ctypes.c_float]
+    for name in ("sin", "cos"):
+        getattr(libm, name).restype = ctypes.c_double
+        getattr(libm, name).argtypes = [ctypes.c_double]
+    rng = np.random.default_rng(3)
+    yaw = rng.uniform(-np.pi, np.pi, 50000).astype(F32)
+    # glibc's double sin/cos rounded once (numpy's own float64 sin may differ)
+    sin_d = np.array([libm.sin(float(a)) for a in yaw], np.float64).astype(F32)
+    cos_d = np.array([libm.cos(float(a)) for a in yaw], np.float64).astype(F32)
+    sin_f = np.array([libm.sinf(float(a)) for a in yaw], F32)
+    cos_f = np.array([libm.cosf(float(a)) for a in yaw], F32)
+    rate_sin = float((sin_d.view(np.int32) != sin_f.view(np.int32)).mean())
+    rate_cos = float((cos_d.view(np.int32) != cos_f.view(np.int32)).mean())
+    cells = weights = indices = total_w = 0
+    for seed in range(1, 6):
+        c = W.room_case(seed)
+        bm = c["map"]
+        _, p0 = oracle.compute_grid(bm.points(), bm.min, bm.max, bm.res, c["sensor_dev"], exp_mode=0)
+        _, p1 = oracle.compute_grid(bm.points(), bm.min, bm.max, bm.res, c["sensor_dev"], exp_mode=1)
+        cells += int((p0.view(np.int32) != p1.view(np.int32)).sum())
+        s, parts = c["step"], c["particles"]
+        out = []
+        for prob in (p0, p1):
+            g = oracle.GridMeta(bm.min, bm.max, bm.res, prob)
+            op = oracle.pf_predict(parts, s.mods, s.deltas, s.z4)
+            op, _ = oracle.pf_update(op, g, s.cloud, s.ranges, s.alpha, s.sigma)
+            out.append((op[:, 4].copy(), oracle.resample_indices(op[:, 4], s.u01, 0)))
+        weights += int((out[0][0].view(np.int32) != out[1][0].view(np.int32)).sum())
+        indices += int((out[0][1] != out[1][1]).sum())
+        total_w += parts.shape[0]
+    print("libm: sinf vs (float)sin(double) differ on %.2e of yaws, cosf on %.2e; "
+          "expf vs (float)exp(double): %d of %d room cells, %d of %d weights, %d resample indices"
+          % (rate_sin, rate_cos, cells, 5 * 2400000, weights, total_w, indices))
+    assert cells < 1e-3 * 5 * 2400000
