@@ -614,7 +614,7 @@ struct SubRec {
   uint32_t d0a, d1a, d0b, d1b;
   uint32_t pp;  // p0a | p1a << 1 | p0b << 2 | p1b << 3
   int32_t e;
-  uint32_t flags;  // kNoMapA, kNoMapB, kPos, kNeg
+  uint32_t flags;  // kNoMapA, kNoMapB, kPos, kNeg, kIrregularTerm (a NaN term)
   uint32_t pad;
 };
 static_assert(sizeof(SubRec) == 32, "8 floats per sub-tile record");
@@ -670,8 +670,9 @@ __device__ __forceinline__ ParityFn term_map(float x, double inv_u, bool& hard) 
   return f;
 }
 
-// The composition of the 64 lanes' maps in lane order, in lane 63.
-__device__ __forceinline__ ParityFn wave_total_fn(ParityFn f) {
+// The composition of the maps of lanes 0..l in lane order, in lane l (lane
+// 63: the wavefront's).
+__device__ __forceinline__ ParityFn wave_inclusive_fn(ParityFn f) {
   ParityFn inc = row_inclusive_fn(f);
   inc = fn_then(dpp_fn<0x142, 0xa>(inc), inc);
   inc = fn_then(dpp_fn<0x143, 0xc>(inc), inc);
@@ -759,9 +760,10 @@ __global__ __launch_bounds__(kChainThreads) void tiled_map_kernel(Filters fs, in
     ma = term_map(xi, ldexp(1.0, 23 - es), hard_a);
     mb = term_map(xi, ldexp(1.0, 22 - es), hard_b);
   }
-  const ParityFn ta = wave_total_fn(ma), tb = wave_total_fn(mb);
+  const ParityFn ta = wave_inclusive_fn(ma), tb = wave_inclusive_fn(mb);
   const bool sub_hard_a = __ballot(hard_a) != 0, sub_hard_b = __ballot(hard_b) != 0;
   const bool sub_pos = __ballot(xi > 0.0f) != 0, sub_neg = __ballot(xi < 0.0f) != 0;
+  const bool sub_nan = __ballot(xi != xi) != 0;
   if (lane == 63) {
     SubRec* S = tiled_subs(f, k) + (size_t)t * kSubTiles + wave;
     S->d0a = ta.d0;
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(kChainThreads) void tiled_map_kernel(Filters fs, in
     S->pp = ta.p0 | (ta.p1 << 1) | (tb.p0 << 2) | (tb.p1 << 3);
     S->e = es;
     S->flags = (sub_hard_a ? kNoMapA : 0u) | (sub_hard_b ? kNoMapB : 0u) | (sub_pos ? kPos : 0u) |
-               (sub_neg ? kNeg : 0u);
+               (sub_neg ? kNeg : 0u) | (sub_nan ? kIrregularTerm : 0u);
   }
   const int any_hard = __syncthreads_or(hard ? 1 : 0);
   const ParityFn ex = block_exclusive_fn(m, L);
@@ -790,120 +792,177 @@ __global__ __launch_bounds__(kChainThreads) void tiled_map_kernel(Filters fs, in
   }
 }
 
-// s + (the run's increment of k under binade e): the new sum, or false when
-// the run does not stay in the binade of s (or s is not a normal number of
-// the terms' sign).
-__device__ __forceinline__ bool apply_map(float s, uint32_t flags, int e, uint32_t d0, uint32_t d1, float& out) {
-  if (!(fabsf(s) >= 0x1p-126f) || !isfinite(s) || float_exponent(fabsf(s)) != e) return false;
-  if ((s > 0.0f) ? (flags & kNeg) : (flags & kPos)) return false;
-  const uint32_t kin = (uint32_t)(fabs((double)s) * ldexp(1.0, 23 - e));  // in [2^23, 2^24)
-  const uint64_t kout = (uint64_t)kin + ((kin & 1u) ? d1 : d0);
-  if (kout >= (1ull << 24)) return false;
-  const float so = (float)((double)kout * ldexp(1.0, e - 23));
-  out = (s < 0.0f) ? -so : so;
-  return true;
+__device__ __forceinline__ uint32_t first_false(bool p) {  // the first lane where p is false (64: none)
+  const uint64_t m = ~(uint64_t)__ballot(p);
+  return m ? (uint32_t)__builtin_ctzll(m) : 64u;
 }
 
+__device__ __forceinline__ float from_k(uint32_t k, int e, bool neg) {
+  const float v = (float)((double)k * ldexp(1.0, e - 23));
+  return neg ? -v : v;
+}
+
+// Maps m, one per lane in order, applied from s = ±kin·2^(e-23) (s normal,
+// in binade e): the number C of leading lanes whose maps apply (all valid,
+// and k stays below 2^24: every term in the binade), with kb and ka, k
+// before and after each lane's map.  A valid map adds less than 2^24, so 64
+// of them cannot wrap k.
+__device__ __forceinline__ uint32_t run_maps(float s, int e, ParityFn m, bool valid, uint32_t& kb, uint32_t& ka) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t L = first_false(valid);
+  if (!valid) m = ParityFn{0u, 0u, 0u, 1u};
+  const ParityFn inc = wave_inclusive_fn(m);
+  const uint32_t kin = (uint32_t)(fabs((double)s) * ldexp(1.0, 23 - e));  // in [2^23, 2^24)
+  ka = kin + ((kin & 1u) ? inc.d1 : inc.d0);
+  kb = (uint32_t)__shfl_up((int)ka, 1);
+  if (lane == 0) kb = kin;
+  return first_false(lane < L && ka < (1u << 24));
+}
+
+__device__ __forceinline__ bool normal_finite(float s) { return fabsf(s) >= 0x1p-126f && isfinite(s); }
+
 struct WalkLds {
-  uint32_t d0[kWalkTiles], d1[kWalkTiles], fe[kWalkTiles];  // fe: flags | (e + 256) << 8
+  // per tile: the map's increments, and fe = flags | pp << 8 | (e + 256) << 10
+  uint32_t d0[kWalkTiles], d1[kWalkTiles], fe[kWalkTiles];
   float s_in[kWalkTiles];
   uint32_t regular[kWalkTiles];
   float s;
-  uint32_t stop, run_end, irr;
+  uint32_t stop, run_end;
 };
 
-// Tile t sub-tile by sub-tile (wavefront 0, every lane alike): s after it;
-// the checkpoint after every sub-tile goes to prefix (when given).
+// Tile t sub-tile by sub-tile (wavefront 0, every lane alike), from s of +0
+// or of the sign of the tile's terms: runs of sub-tiles whose maps apply in
+// one wavefront scan, any other sub-tile as 64 float adds in order; the
+// checkpoint after every sub-tile goes to prefix (when given).
 __device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, const SubRec* subs, float* prefix) {
-  const int lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63;
   SubRec r{};
-  if (lane < (int)kSubTiles) r = subs[(size_t)t * kSubTiles + lane];
-  for (uint32_t q = 0; q < kSubTiles; ++q) {
-    const uint32_t base = t * kTiledTile + q * 64;
-    if (base >= n) break;
-    const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)r.flags, q);
-    const int e = __builtin_amdgcn_readlane(r.e, q);
-    bool done = false;
-    if (s == 0.0f && !(flags & (kPos | kNeg))) {
-      done = true;  // +0 plus zeros stays +0
-    } else {
-      float so;
-      if (!(flags & kNoMapA) &&
-          apply_map(s, flags, e, (uint32_t)__builtin_amdgcn_readlane((int)r.d0a, q),
-                    (uint32_t)__builtin_amdgcn_readlane((int)r.d1a, q), so)) {
-        s = so;
-        done = true;
-      } else if (!(flags & kNoMapB) &&
-                 apply_map(s, flags, e + 1, (uint32_t)__builtin_amdgcn_readlane((int)r.d0b, q),
-                           (uint32_t)__builtin_amdgcn_readlane((int)r.d1b, q), so)) {
-        s = so;
-        done = true;
+  if (lane < kSubTiles) r = subs[(size_t)t * kSubTiles + lane];
+  const uint32_t t0 = t * kTiledTile;
+  const uint32_t nq = min(kSubTiles, (n - t0 + 63) / 64);  // the sub-tiles holding terms
+  for (uint32_t q = 0; q < nq;) {
+    const uint32_t Q = q + lane;  // this lane's sub-tile
+    const bool have = Q < nq;
+    const int src = have ? (int)Q : 0;
+    const uint32_t fl = (uint32_t)__shfl((int)r.flags, src);
+    if (s == 0.0f) {  // +0 plus sub-tiles of zeros stays +0
+      const uint32_t Z = first_false(have && !(fl & (kPos | kNeg | kIrregularTerm)));
+      if (prefix && lane < Z) prefix[(t0 + Q * 64) / kGroup] = 0.0f;
+      q += Z;
+      if (Z > 0) continue;
+    } else if (normal_finite(s)) {
+      const int E = float_exponent(fabsf(s));
+      const int e = __shfl(r.e, src);
+      const uint32_t pp = (uint32_t)__shfl((int)r.pp, src);
+      // every lane shuffles (a lane reads only lanes that run the shuffle)
+      const uint32_t d0a = (uint32_t)__shfl((int)r.d0a, src), d1a = (uint32_t)__shfl((int)r.d1a, src);
+      const uint32_t d0b = (uint32_t)__shfl((int)r.d0b, src), d1b = (uint32_t)__shfl((int)r.d1b, src);
+      const bool use_a = e == E && !(fl & kNoMapA);
+      const bool use_b = e + 1 == E && !(fl & kNoMapB);
+      const ParityFn m = use_a ? ParityFn{d0a, d1a, pp & 1u, (pp >> 1) & 1u}
+                               : ParityFn{d0b, d1b, (pp >> 2) & 1u, (pp >> 3) & 1u};
+      const bool sign_ok = (s > 0.0f) ? !(fl & kNeg) : !(fl & kPos);
+      const bool valid = have && (use_a || use_b) && sign_ok && m.d0 < (1u << 24) && m.d1 < (1u << 24);
+      uint32_t kb, ka;
+      const uint32_t C = run_maps(s, E, m, valid, kb, ka);
+      if (C > 0) {
+        const bool neg = s < 0.0f;
+        if (prefix && lane < C) prefix[(t0 + Q * 64) / kGroup] = from_k(ka, E, neg);
+        s = from_k((uint32_t)__builtin_amdgcn_readlane((int)ka, C - 1), E, neg);
+        q += C;
+        continue;
       }
     }
-    if (!done) {  // the reference's float adds, in order
-      const uint32_t len = min(64u, n - base);
-      const float xv = ((uint32_t)lane < len) ? x[base + lane] : 0.0f;
-      for (uint32_t j = 0; j < len; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), j));
-    }
+    // sub-tile q by the reference's float adds, in order
+    const uint32_t base = t0 + q * 64, len = min(64u, n - base);
+    const float xv = lane < len ? x[base + lane] : 0.0f;
+    for (uint32_t j = 0; j < len; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), j));
     if (prefix && lane == 0) prefix[base / kGroup] = s;
+    ++q;
   }
   return s;
 }
 
-// One workgroup per sum walks the tiles in order.
+// One workgroup per sum walks the tiles in order.  Wavefront 0 takes runs of
+// tiles whose maps apply in one wavefront scan each (so a run of up to 64
+// tiles in one binade costs one scan), a tile that cannot be taken whole
+// (a binade crossing inside it, no map) sub-tile by sub-tile, and stops at a
+// tile whose terms have both signs or the other sign than s, or at a sum
+// that is not finite: the whole workgroup then runs exact_sum_range.
 __global__ __launch_bounds__(kChainThreads) void tiled_walk_kernel(Filters fs, int which) {
   __shared__ ExactLds L;
   __shared__ WalkLds W;
   const FilterDev& f = fs.one;
-  const uint32_t n = f.n, k = blockIdx.y, tid = threadIdx.x;
+  const uint32_t n = f.n, k = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
   const float* x = tiled_src(f, which, k);
   float* prefix = tiled_has_prefix(which, k) ? f.scratch + sc_c32(n) : nullptr;
   const uint32_t T = (uint32_t)tiled_tiles(n);
   TileRec* recs = tiled_recs(f, k);
   const SubRec* subs = tiled_subs(f, k);
   float s = 0.0f;
-  if (tid == 0) W.irr = 0u;
+  int irr = 0;
   for (uint32_t c0 = 0; c0 < T; c0 += kWalkTiles) {  // a chunk of tile records in LDS
     const uint32_t c1 = min(T, c0 + kWalkTiles);
+    uint32_t fl_any = 0u;
     for (uint32_t j = c0 + tid; j < c1; j += kChainThreads) {
       const TileRec& r = recs[j];
       W.d0[j - c0] = r.d0;
       W.d1[j - c0] = r.d1;
-      W.fe[j - c0] = r.flags | ((uint32_t)(r.e + 256) << 8);
+      W.fe[j - c0] = r.flags | (r.pp << 8) | ((uint32_t)(r.e + 256) << 10);
       W.regular[j - c0] = 0u;
+      fl_any |= r.flags;
     }
-    __syncthreads();
+    irr |= __syncthreads_or((fl_any & kIrregularTerm) ? 1 : 0);
     for (uint32_t t = c0; t < c1;) {  // uniform over the workgroup
       if (tid < 64) {
-        // wavefront 0 walks: whole tiles by their maps, others sub-tile by
-        // sub-tile, until a tile with terms of both signs
-        uint32_t irr = W.irr;
-        for (; t < c1; ++t) {
-          const uint32_t j = t - c0, fl = W.fe[j] & 0xffu;
-          const int e = (int)(W.fe[j] >> 8) - 256;
-          if (tid == 0) W.s_in[j] = s;
-          irr |= fl & kIrregularTerm;
-          if (s == 0.0f && !(fl & (kPos | kNeg))) {  // +0 plus zeros: checkpoints of +0
-            if (tid == 0) W.regular[j] = 1u;
-            continue;
+        uint32_t end = c1;  // where a stop's exact_sum_range ends
+        while (t < c1) {
+          const uint32_t j = t - c0 + lane;
+          const bool have = t + lane < c1;
+          const uint32_t fe = have ? W.fe[j] : (uint32_t)kNoMap;
+          const uint32_t fl = fe & 0xffu;
+          if (s == 0.0f) {  // +0 plus tiles of zeros stays +0
+            const uint32_t Z = first_false(have && !(fl & (kPos | kNeg | kIrregularTerm)));
+            if (lane < Z) {
+              W.s_in[j] = 0.0f;
+              W.regular[j] = 1u;
+            }
+            t += Z;
+            if (Z > 0) continue;
+          } else if (normal_finite(s)) {
+            const int E = float_exponent(fabsf(s));
+            const bool sign_ok = (s > 0.0f) ? !(fl & kNeg) : !(fl & kPos);
+            const ParityFn m{have ? W.d0[j] : 0u, have ? W.d1[j] : 0u, (fe >> 8) & 1u, (fe >> 9) & 1u};
+            const bool valid = have && !(fl & kNoMap) && (int)(fe >> 10) - 256 == E && sign_ok &&
+                               m.d0 < (1u << 24) && m.d1 < (1u << 24);
+            uint32_t kb, ka;
+            const uint32_t C = run_maps(s, E, m, valid, kb, ka);
+            if (C > 0) {
+              const bool neg = s < 0.0f;
+              if (lane < C) {
+                W.s_in[j] = from_k(kb, E, neg);
+                W.regular[j] = 1u;
+              }
+              s = from_k((uint32_t)__builtin_amdgcn_readlane((int)ka, C - 1), E, neg);
+              t += C;
+              continue;
+            }
+          } else if (!isfinite(s)) {
+            break;  // the rest of the chunk by exact_sum_range
           }
-          float so;
-          if (!(fl & kNoMap) && apply_map(s, fl, e, W.d0[j], W.d1[j], so)) {
-            s = so;
-            if (tid == 0) W.regular[j] = 1u;
-            continue;
+          // tile t cannot be taken whole
+          const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)fl);
+          const bool mixed = (f0 & kPos) && (f0 & kNeg);
+          const bool other_sign = (s > 0.0f && (f0 & kNeg)) || (s < 0.0f && (f0 & kPos));
+          if (mixed || other_sign) {  // then while the tiles' terms have both signs
+            end = t + 1;
+            while (end < c1 && (W.fe[end - c0] & kPos) && (W.fe[end - c0] & kNeg)) ++end;
+            break;
           }
-          if ((fl & kPos) && (fl & kNeg)) break;
           s = walk_subtiles(s, t, n, x, subs, prefix);
-        }
-        // the tiles from the one that stopped it while their terms have both signs
-        uint32_t end = t < c1 ? t + 1 : c1;
-        while (end < c1 && (W.fe[end - c0] & kPos) && (W.fe[end - c0] & kNeg)) {
-          irr |= W.fe[end - c0] & kIrregularTerm;
-          ++end;
+          ++t;
         }
         if (tid == 0) {
-          W.irr = irr;
           W.s = s;
           W.stop = t;
           W.run_end = end;
@@ -914,7 +973,7 @@ __global__ __launch_bounds__(kChainThreads) void tiled_walk_kernel(Filters fs, i
       const uint32_t stop = W.stop, end = W.run_end;
       __syncthreads();
       if (stop < c1) s = exact_sum_range(s, stop * kTiledTile, min(n, end * kTiledTile), n, x, prefix, L);
-      t = end;
+      t = stop < c1 ? end : c1;
     }
     for (uint32_t j = c0 + tid; j < c1; j += kChainThreads) {
       recs[j].s_in = W.s_in[j - c0];
@@ -924,7 +983,7 @@ __global__ __launch_bounds__(kChainThreads) void tiled_walk_kernel(Filters fs, i
   }
   if (tid == 0) {
     *tiled_dst(f, which, k) = s;
-    if (prefix) sums_of(f)[4] = W.irr ? 1.0f : 0.0f;
+    if (prefix) sums_of(f)[4] = irr ? 1.0f : 0.0f;
   }
 }
 

@@ -57,11 +57,13 @@ def test_tiled_full_step(gpu, room, n, yaw, shift):
     assert np.array_equal(pf.particles().view(np.int32), oq.view(np.int32))
 
 
-@pytest.mark.parametrize("kind", ["exponential", "growing", "ties", "zeros_first", "subnormal", "one_hot"])
+@pytest.mark.parametrize("kind", ["exponential", "growing", "ties", "zeros_first", "subnormal", "one_hot", "nan_after_zeros",
+                                  "inf_mid", "negative_run"])
 def test_tiled_resample_running_sum(gpu, kind):
     """resample alone (its running sum over the set weights, tiled) against
     the reference's loop, for weights that cross many binades, sit on exact
-    ties, start with long zero runs or stay subnormal."""
+    ties, start with long zero runs, stay subnormal, hold a NaN or an inf, or
+    change sign for a run."""
     n = 70000
     rng = np.random.default_rng(3)
     if kind == "exponential":
@@ -74,9 +76,18 @@ def test_tiled_resample_running_sum(gpu, kind):
         w = np.concatenate([np.zeros(30000), rng.uniform(0, 1, n - 30000)])
     elif kind == "subnormal":
         w = rng.uniform(0, 1, n) * 1e-41
-    else:
+    elif kind == "one_hot":
         w = np.zeros(n)
         w[54321] = 1.0
+    elif kind == "nan_after_zeros":
+        w = np.concatenate([np.zeros(30000), rng.uniform(0, 1, n - 30000)])
+        w[30000] = np.nan
+    elif kind == "inf_mid":
+        w = rng.uniform(0, 1, n)
+        w[40000] = np.inf
+    else:  # a run of negative weights between positive ones
+        w = rng.uniform(0, 1, n)
+        w[20000:26000] *= -1.0
     p = np.zeros((n, 7), F32)
     p[:, 0] = np.arange(n)
     p[:, 4] = w.astype(F32)
