@@ -43,6 +43,10 @@ PROFILE_TABLE = os.path.join(ROOT, "profiles", "cloud_weight_large_filter_per_up
 # The chip's vector issue rate: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64
 # instruction.
 ISSUE_PEAK = 1024 * 2.4e9 / 4
+# The shader clock the same kernel ran at, per update (rocprofv3 --pmc
+# GRBM_GUI_ACTIVE over the kernel trace's duration, one run of the same
+# workload and shape).
+CLOCK_TABLE = os.path.join(ROOT, "profiles", "cloud_weight_large_filter_clock.md")
 
 
 def log(*a):
@@ -191,7 +195,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": counters.get("traffic"),
             "traffic_note": counters["note"],
-            "issue": issue_block(counters, evals_per_launch, weight_ms),
+            "issue": issue_block(counters, evals_per_launch, weight_ms, profile_clock(CLOCK_TABLE, Wm, K)),
             "kernel_ms_avg": weight_ms,
             "kernel_ms_profiled": counters.get("kernel_ms"),
             "kernel_launches": launches,
@@ -269,16 +273,41 @@ def profiled_shape(path):
     return None
 
 
-def issue_block(counters, evals_per_launch, weight_ms):
+def profile_clock(path, warmup, steps):
+    """Mean shader clock (GHz) over updates warmup+1 .. warmup+steps of the
+    clock table, or None when it does not cover them."""
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    clk = {}
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0].isdigit():
+            clk[int(cells[0])] = float(cells[3])
+    upd = range(warmup + 1, warmup + steps + 1)
+    if not all(u in clk for u in upd):
+        return None
+    return sum(clk[u] for u in upd) / len(upd)
+
+
+def issue_block(counters, evals_per_launch, weight_ms, clock_ghz=None):
     if "valu_per_64_evals" not in counters:
         return None
     v = counters["valu_per_64_evals"]
     rate = evals_per_launch / 64 * v / (weight_ms * 1e-3)
-    return {"valu_instr_per_64_evals": v, "achieved_wave_instr_per_s": rate, "peak_wave_instr_per_s": ISSUE_PEAK,
-            "frac": rate / ISSUE_PEAK,
-            "note": "SQ_INSTS_VALU per dispatch from the same profile table, over the same updates; the kernel is "
-                    "bound by vector instruction issue, not HBM: per evaluation the reference's FP64 offset add, "
-                    "float rounding, bounds and cell arithmetic per axis"}
+    out = {"valu_instr_per_64_evals": v, "achieved_wave_instr_per_s": rate, "peak_wave_instr_per_s": ISSUE_PEAK,
+           "frac": rate / ISSUE_PEAK,
+           "note": "SQ_INSTS_VALU per dispatch from the same profile table, over the same updates; the kernel is "
+                   "bound by vector instruction issue, not HBM: per evaluation the reference's FP64 offset add, "
+                   "float rounding, bounds and cell arithmetic per axis"}
+    if clock_ghz:
+        out["clock_ghz_profiled"] = clock_ghz
+        out["frac_at_profiled_clock"] = rate / (1024 * clock_ghz * 1e9 / 4)
+        out["clock_note"] = ("the shader clock the kernel ran at over the same updates (GRBM_GUI_ACTIVE per XCD over "
+                             "the traced duration, %s); frac_at_profiled_clock is the issue rate over what the SIMDs "
+                             "issue at that clock" % os.path.relpath(CLOCK_TABLE, ROOT))
+    return out
 
 
 def choose_shape(amcl3d_amd, pf, grid, bufs, args, alpha, sigma):
