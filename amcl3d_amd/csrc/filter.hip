@@ -826,6 +826,7 @@ struct WalkLds {
   uint32_t d0[kWalkTiles], d1[kWalkTiles], fe[kWalkTiles];
   float s_in[kWalkTiles];
   uint32_t regular[kWalkTiles];
+  float xt[kTiledTile];  // the terms of the tile being walked sub-tile by sub-tile (0 past n)
   float s;
   uint32_t stop, run_end;
 };
@@ -834,11 +835,21 @@ struct WalkLds {
 // or of the sign of the tile's terms: runs of sub-tiles whose maps apply in
 // one wavefront scan, any other sub-tile as 64 float adds in order; the
 // checkpoint after every sub-tile goes to prefix (when given).
-__device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, const SubRec* subs, float* prefix) {
+__device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, const SubRec* subs, float* prefix,
+                               float* xt) {
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t t0 = t * kTiledTile;
+  // the sub-tile records and every term of the tile, loaded together
   SubRec r{};
   if (lane < kSubTiles) r = subs[(size_t)t * kSubTiles + lane];
-  const uint32_t t0 = t * kTiledTile;
+  float xv[kSubTiles];
+#pragma unroll
+  for (uint32_t i = 0; i < kSubTiles; ++i) {
+    const uint32_t idx = t0 + i * 64 + lane;
+    xv[i] = idx < n ? x[idx] : 0.0f;
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kSubTiles; ++i) xt[i * 64 + lane] = xv[i];  // only this wavefront reads them
   const uint32_t nq = min(kSubTiles, (n - t0 + 63) / 64);  // the sub-tiles holding terms
   for (uint32_t q = 0; q < nq;) {
     const uint32_t Q = q + lane;  // this lane's sub-tile
@@ -873,10 +884,19 @@ __device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, 
         continue;
       }
     }
-    // sub-tile q by the reference's float adds, in order
-    const uint32_t base = t0 + q * 64, len = min(64u, n - base);
-    const float xv = lane < len ? x[base + lane] : 0.0f;
-    for (uint32_t j = 0; j < len; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xv), j));
+    // sub-tile q by the reference's float adds, in order: every lane reads
+    // the same terms from LDS (a broadcast) and adds them (the zeros past n
+    // leave the sum, never -0.0, unchanged)
+    const uint32_t base = t0 + q * 64;
+    const float4* xq = reinterpret_cast<const float4*>(xt + q * 64);
+#pragma unroll
+    for (int c = 0; c < 16; c += 4) {
+      const float4 a = xq[c], b = xq[c + 1], d = xq[c + 2], e = xq[c + 3];
+      s += a.x; s += a.y; s += a.z; s += a.w;
+      s += b.x; s += b.y; s += b.z; s += b.w;
+      s += d.x; s += d.y; s += d.z; s += d.w;
+      s += e.x; s += e.y; s += e.z; s += e.w;
+    }
     if (prefix && lane == 0) prefix[base / kGroup] = s;
     ++q;
   }
@@ -959,7 +979,7 @@ __global__ __launch_bounds__(kChainThreads) void tiled_walk_kernel(Filters fs, i
             while (end < c1 && (W.fe[end - c0] & kPos) && (W.fe[end - c0] & kNeg)) ++end;
             break;
           }
-          s = walk_subtiles(s, t, n, x, subs, prefix);
+          s = walk_subtiles(s, t, n, x, subs, prefix, W.xt);
           ++t;
         }
         if (tid == 0) {
