@@ -95,3 +95,46 @@ def test_tiled_resample_running_sum(gpu, kind):
     pf.set_particles(p)
     idx = pf.resample(0.41, gpu.RESAMPLE_REFERENCE, return_indices=True)
     assert np.array_equal(idx, oracle.resample_indices(p[:, 4], 0.41, 0)), kind
+
+
+@pytest.mark.parametrize("n", [1_048_576 + 1, 2_500_000])
+def test_tiled_resample_past_one_chunk_of_tile_records(gpu, n):
+    """More than 1,024 tiles (the walk holds the tile records of one chunk
+    in LDS at a time): resample indices over several chunks, weights growing
+    across binades, against the reference's loop."""
+    rng = np.random.default_rng(n)
+    w = (np.exp(np.linspace(-20, 5, n)) * rng.uniform(0.5, 1.5, n)).astype(F32)
+    p = np.zeros((n, 7), F32)
+    p[:, 0] = np.arange(n) % 1000
+    p[:, 4] = w
+    pf = gpu.ParticleFilter()
+    pf.set_particles(p)
+    idx = pf.resample(0.37, gpu.RESAMPLE_REFERENCE, return_indices=True)
+    assert np.array_equal(idx, oracle.resample_indices(w, 0.37, 0))
+
+
+def test_tiled_full_step_past_one_chunk(gpu, room):
+    """A full step of 1.2M particles (1,172 tiles, two chunks of tile records)
+    with a 200-point cloud: every sum of the update tail and resample bit for
+    bit against the oracle."""
+    c, bm = room["case"], room["bm"]
+    n = 1_200_000
+    rng = np.random.default_rng(12)
+    s = c["step"]
+    grid = gpu.Grid3d().from_cells(room["prob"], bm.min, bm.max, bm.res, c["sensor_dev"])
+    og = oracle.GridMeta(bm.min, bm.max, bm.res, room["prob"])
+    parts = W.particles_around(np.array([10.0, 10.0, 1.5, 0.2]), n, rng)
+    z4 = rng.standard_normal((n, 4)).astype(F32)
+    cloud = s.cloud[:200]
+    pf = gpu.ParticleFilter()
+    pf.set_particles(parts)
+    pf.predict(*s.mods, *s.deltas, z4)
+    pf.update(grid, cloud, s.ranges, s.alpha, s.sigma)
+    upd, mean = pf.particles(), pf.getMean()
+    idx = pf.resample(s.u01, gpu.RESAMPLE_REFERENCE, return_indices=True)
+    op = oracle.pf_predict(parts, s.mods, s.deltas, z4)
+    op, om = oracle.pf_update(op, og, cloud, s.ranges, s.alpha, s.sigma)
+    _, oidx = oracle.pf_resample(op, s.u01, 0)
+    assert np.array_equal(upd.view(np.int32), op.view(np.int32))
+    assert np.array_equal(mean.view(np.int32), om.view(np.int32))
+    assert np.array_equal(idx, oidx)
