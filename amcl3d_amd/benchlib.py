@@ -141,7 +141,7 @@ def _timed(fn):
 
 def served(amcl3d_amd, W, maps, nsessions=256, rate_hz=10.0, duration_s=8.0, seed=5, max_inflight=3,
            max_batch_evals=800_000_000, latest_only=True, poses_per_map=12, n_range=(1000, 50000),
-           m_range=(2000, 20000), slo_ms=None, ladder=(0.9, 0.75, 0.6, 0.45, 0.3, 0.15), slo_duration_s=4.0):
+           m_range=(2000, 20000), slo_ms=None, ladder=(1.0, 0.95, 0.9, 0.8, 0.7, 0.6, 0.45, 0.3, 0.15), slo_duration_s=4.0):
     """Sessions sending Poisson update requests at rate_hz each.
 
     maps: list of (BoxMap view for ray casting, Grid3d, sensor height z).
