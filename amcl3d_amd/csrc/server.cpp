@@ -36,6 +36,10 @@ using namespace a3d;
 
 namespace {
 
+// Under a backlog a batch may grow up to this many times the configured
+// bound (max_batch_evals).
+constexpr uint64_t kBacklogGrowth = 8;
+
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
@@ -383,6 +387,19 @@ void a3d_server::run() {
         progress = true;
       }
     }
+    // the batch bound: the configured one, or under a backlog the queued
+    // evaluations shared among the free slots (at most kBacklogGrowth x)
+    uint64_t bound = cfg.max_batch_evals;
+    {
+      int free_slots = 0;
+      for (auto& s : slots) free_slots += s.busy ? 0 : 1;
+      if (free_slots > 0 && !pending.empty()) {
+        uint64_t backlog = 0;
+        for (auto& r : pending)
+          if (!sessions[r.session].busy) backlog += (uint64_t)sessions[r.session].n * r.m;
+        bound = std::max(bound, std::min(kBacklogGrowth * cfg.max_batch_evals, backlog / (uint64_t)free_slots));
+      }
+    }
     for (auto& s : slots) {
       if (s.busy || pending.empty()) continue;
       // form a batch in arrival order: one request per session, bounded size
@@ -393,7 +410,7 @@ void a3d_server::run() {
         bool dup = se.busy;
         if (!dup) {
           const uint64_t e = (uint64_t)se.n * it->m;
-          if (!s.reqs.empty() && evals + e > cfg.max_batch_evals) break;
+          if (!s.reqs.empty() && evals + e > bound) break;
           evals += e;
           se.busy = true;
           s.reqs.push_back(std::move(*it));

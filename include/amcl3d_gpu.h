@@ -315,7 +315,11 @@ int a3d_fleet_capture_end(a3d_fleet* fl, a3d_graph** out);
  * depend on which other sessions share its batches. */
 typedef struct {
   size_t hbm_budget_bytes;    /* session state + staging admitted at most */
-  uint64_t max_batch_evals;   /* particle.point evaluations per launch */
+  uint64_t max_batch_evals;   /* particle.point evaluations per launch; under a
+                                 backlog the bound grows to the queued
+                                 evaluations over the free launch slots, up to
+                                 8x this, so an overloaded server runs fewer,
+                                 larger launches */
   int max_batch_sessions;     /* sessions per launch */
   int max_inflight;           /* launches in flight (streams) */
   int latest_only;            /* != 0: a session keeps at most one queued request; a
