@@ -13,7 +13,7 @@ Grid3d::computeCloudWeight, amcl3d/src/Grid3d.cpp:168-196), counted for
 in-map particles only.
 
 Prints one JSON line.  Besides the timed graph it runs, outside the timed
-region: the same steps eagerly with HIP events around every cloud-weight
+region: the same steps eagerly, back to back, with HIP events around every cloud-weight
 launch (roofline), a bit-exact comparison of the two runs, the oracle on a
 bounded sample of the same step on one host core (cpu_baseline), and the
 batched / served forms on their own workloads (extras).
@@ -133,10 +133,10 @@ def main():
     graph_state = pf.particles()
     graph_mean = pf.getMean()
 
-    # the same steps eagerly, HIP events around each cloud-weight launch;
-    # particles are read back after each step to count in-map particles
+    # the same steps eagerly, particles read back after each predict to count
+    # the in-map particles; then again back to back (no host wait between
+    # steps, as in the graph) with HIP events around each cloud-weight launch
     pf.set_particles(start_state)
-    pf.timing(reset=True)
     inmap_evals = 0
     evals_per_update = []
     bm = wl["bm"]
@@ -150,6 +150,12 @@ def main():
         evals_per_update.append(int(inside.sum()) * args.points)
         pf.update_dev(grid, x.ptr, y.ptr, z.ptr, args.points, r.ptr, args.beacons, alpha, sigma)
         pf.resample_dev(wl["u01"][k])
+    pf.synchronize()
+    pf.set_particles(start_state)
+    pf.timing(reset=True)
+    for k in range(Wm, Wm + K):
+        enqueue(k)
+    pf.synchronize()
     weight_ms, launches, _ = pf.timing(reset=True)
     eager_state = pf.particles()
     identical = bool(np.array_equal(eager_state.view(np.int32), graph_state.view(np.int32)) and
