@@ -699,13 +699,12 @@ struct ScanLds {
   uint32_t d0[kChainThreads / 64], d1[kChainThreads / 64], pp[kChainThreads / 64];
 };
 
-// One workgroup per (tile, sum), one term per thread.
-__global__ __launch_bounds__(kChainThreads) void tiled_dsum_kernel(Filters fs, int which) {
-  __shared__ double red[kChainThreads / 64];
-  const FilterDev& f = fs.one;
-  const uint32_t n = f.n, t = blockIdx.x, k = blockIdx.y;
-  const uint32_t i = t * kTiledTile + threadIdx.x;
-  const float xi = i < n ? tiled_src(f, which, k)[i] : 0.0f;
+// Tile t's record of sum k from its terms, one per thread of the workgroup
+// (xi = 0 past n): the double sums of the terms and of their magnitudes, and
+// their signs.
+__device__ __forceinline__ void tile_dsum(const FilterDev& f, uint32_t k, uint32_t t, uint32_t i, float xi,
+                                          double* red) {
+  const uint32_t n = f.n;
   const double ds = block_sum_d((double)xi, red);
   const double as = block_sum_d(fabs((double)xi), red);
   const int pos = __syncthreads_or(xi > 0.0f ? 1 : 0);
@@ -717,6 +716,15 @@ __global__ __launch_bounds__(kChainThreads) void tiled_dsum_kernel(Filters fs, i
     R->asum = as;
     R->flags = (pos ? kPos : 0u) | (neg ? kNeg : 0u) | (irr ? kIrregularTerm : 0u);
   }
+}
+
+// One workgroup per (tile, sum), one term per thread.
+__global__ __launch_bounds__(kChainThreads) void tiled_dsum_kernel(Filters fs, int which) {
+  __shared__ double red[kChainThreads / 64];
+  const FilterDev& f = fs.one;
+  const uint32_t t = blockIdx.x, k = blockIdx.y;
+  const uint32_t i = t * kTiledTile + threadIdx.x;
+  tile_dsum(f, k, t, i, i < f.n ? tiled_src(f, which, k)[i] : 0.0f, red);
 }
 
 __device__ inline bool normal_guess(double v, int& e) {
@@ -1035,9 +1043,10 @@ __global__ __launch_bounds__(kChainThreads) void tiled_ckpt_kernel(Filters fs, i
 
 inline bool tiled_filter(const Filters& fs) { return !fs.arr && fs.one.n >= kTiledMin; }
 
-void launch_tiled(const Filters& fs, int which, uint32_t nsums, hipStream_t s) {
+// with_dsum false: the kernel that wrote the terms wrote the tile records too.
+void launch_tiled(const Filters& fs, int which, uint32_t nsums, hipStream_t s, bool with_dsum = true) {
   const uint32_t T = (uint32_t)tiled_tiles(fs.one.n);
-  tiled_dsum_kernel<<<dim3(T, nsums), kChainThreads, 0, s>>>(fs, which);
+  if (with_dsum) tiled_dsum_kernel<<<dim3(T, nsums), kChainThreads, 0, s>>>(fs, which);
   tiled_map_kernel<<<dim3(T, nsums), kChainThreads, 0, s>>>(fs, which);
   tiled_walk_kernel<<<dim3(1, nsums), kChainThreads, 0, s>>>(fs, which);
   if (which == kPrefixOnly || which == kMeanPrefix) tiled_ckpt_kernel<<<T, kChainThreads, 0, s>>>(fs, which);
@@ -1110,44 +1119,87 @@ __global__ void init_c_kernel(Filters fs) {
 // ------------------------------------------------------------- update tail
 // ParticleFilter.cpp:129-153: out-of-map particles get w = 0 and add nothing
 // to wtp, wtr (adding +0.0f to a non-negative sum is exact).
-__global__ void tail_a_kernel(GridDev g, Filters fs) {
-  const FilterDev& f = fsel(fs, blockIdx.y);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n) return;
+__device__ __forceinline__ void tail_a_elem(const GridDev& g, const FilterDev& f, uint32_t i, float& tp, float& tr) {
   const bool in = into_map(f.gdev ? *f.gdev : g, f.x[i], f.y[i], f.z[i]);
   f.inmap[i] = in ? 1 : 0;
-  f.scratch[i] = in ? f.wp[i] : 0.f;
-  f.scratch[f.n + i] = in ? f.wr[i] : 0.f;
+  tp = in ? f.wp[i] : 0.f;
+  tr = in ? f.wr[i] : 0.f;
+  f.scratch[i] = tp;
+  f.scratch[f.n + i] = tr;
 }
 
 // ParticleFilter.cpp:158-180
-__global__ void tail_c_kernel(Filters fs) {
-  const FilterDev& f = fsel(fs, blockIdx.y);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n) return;
+__device__ __forceinline__ float tail_c_elem(const FilterDev& f, uint32_t i) {
   const float* s = sums_of(f);
   const float wtp = s[0], wtr = s[1];
   const float wp = (wtp > 0) ? f.wp[i] / wtp : 0.f;
   const float wr = (wtr > 0) ? f.wr[i] / wtr : 0.f;
   f.wp[i] = wp;
   f.wr[i] = wr;
-  f.w[i] = f.inmap[i] ? (float)(wp * f.alpha + wr * (1 - f.alpha)) : 0.f;
+  const float w = f.inmap[i] ? (float)(wp * f.alpha + wr * (1 - f.alpha)) : 0.f;
+  f.w[i] = w;
+  return w;
 }
 
 // ParticleFilter.cpp:182-193: w /= wt and the weighted terms of the mean
-__global__ void tail_e_kernel(Filters fs) {
-  const FilterDev& f = fsel(fs, blockIdx.y);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= f.n) return;
+__device__ __forceinline__ void tail_e_elem(const FilterDev& f, uint32_t i, float (&term)[5]) {
   const uint32_t n = f.n;
   const float wt = sums_of(f)[2];
   const float w = (wt > 0) ? f.w[i] / wt : 0.f;
   f.w[i] = w;
   float* sc = f.scratch;
-  sc[i] = w * f.x[i];
-  sc[n + i] = w * f.y[i];
-  sc[2 * (size_t)n + i] = w * f.z[i];
-  sc[3 * (size_t)n + i] = w * f.a[i];
+  term[0] = w * f.x[i];
+  term[1] = w * f.y[i];
+  term[2] = w * f.z[i];
+  term[3] = w * f.a[i];
+  term[4] = w;
+  for (int k = 0; k < 4; ++k) sc[k * (size_t)n + i] = term[k];
+}
+
+__global__ void tail_a_kernel(GridDev g, Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  float tp, tr;
+  tail_a_elem(g, f, i, tp, tr);
+}
+
+__global__ void tail_c_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  (void)tail_c_elem(f, i);
+}
+
+__global__ void tail_e_kernel(Filters fs) {
+  const FilterDev& f = fsel(fs, blockIdx.y);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.n) return;
+  float term[5];
+  tail_e_elem(f, i, term);
+}
+
+// The same steps for a single filter whose sums run tiled: one workgroup per
+// tile, which also writes the tile's records of the sums that follow (what
+// tiled_dsum_kernel does from terms already written), saving that launch.
+// The mean's step (five sums) keeps its own records kernel, which runs them
+// in parallel workgroups.
+__global__ __launch_bounds__(kChainThreads) void tail_a_tiled_kernel(GridDev g, Filters fs) {
+  __shared__ double red[kChainThreads / 64];
+  const FilterDev& f = fs.one;
+  const uint32_t t = blockIdx.x, i = t * kTiledTile + threadIdx.x;
+  float tp = 0.f, tr = 0.f;
+  if (i < f.n) tail_a_elem(g, f, i, tp, tr);
+  tile_dsum(f, 0, t, i, tp, red);
+  tile_dsum(f, 1, t, i, tr, red);
+}
+
+__global__ __launch_bounds__(kChainThreads) void tail_c_tiled_kernel(Filters fs) {
+  __shared__ double red[kChainThreads / 64];
+  const FilterDev& f = fs.one;
+  const uint32_t t = blockIdx.x, i = t * kTiledTile + threadIdx.x;
+  const float w = i < f.n ? tail_c_elem(f, i) : 0.f;
+  tile_dsum(f, 0, t, i, w, red);
 }
 
 // ---------------------------------------------------------------- resample
@@ -1327,10 +1379,12 @@ void launch_init(const Filters& fs, hipStream_t s) {
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s) {
   if (fs.count == 0 || fs.max_n == 0) return;
   if (tiled_filter(fs)) {
-    tail_a_kernel<<<ew_grid(fs), kEw, 0, s>>>(g, fs);
-    launch_tiled(fs, kTailTotals, 2, s);
-    tail_c_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
-    launch_tiled(fs, kWeightTotal, 1, s);
+    const uint32_t T = (uint32_t)tiled_tiles(fs.one.n);
+    tail_a_tiled_kernel<<<T, kChainThreads, 0, s>>>(g, fs);
+    launch_tiled(fs, kTailTotals, 2, s, false);
+    tail_c_tiled_kernel<<<T, kChainThreads, 0, s>>>(fs);
+    launch_tiled(fs, kWeightTotal, 1, s, false);
+    // (five sums: their records in parallel workgroups, not one after another)
     tail_e_kernel<<<ew_grid(fs), kEw, 0, s>>>(fs);
     launch_tiled(fs, kMeanPrefix, 5, s);
     return;
