@@ -433,3 +433,46 @@ def test_libm_choice_effect_on_room_steps():
           "expf vs (float)exp(double): %d of %d room cells, %d of %d weights, %d resample indices"
           % (rate_sin, rate_cos, cells, 5 * 2400000, weights, total_w, indices))
     assert cells < 1e-3 * 5 * 2400000
+
+
+def test_voxel_sum_order_effect_on_room_steps():
+    """The device sums a voxel's points in point order; PCL sums them in the
+    order its unstable std::sort leaves them (oracle.voxel_grid order_mode 0
+    and 1).  Measured here on full steps of the room map (five seeds, a
+    100,000-point raw scan voxelised at 0.1 m as Node.cpp:130-141 does): the
+    centroid coordinates that differ, and what that does downstream to the
+    particle weights (relative difference) and the resample indices.  A
+    last-bit change of a point can move it to the next cell, so weights can
+    differ by more than the 1e-5 relative that holds for identical inputs:
+    the measured worst is about 2e-4, with no resample index changed; the
+    bounds below are a loose fence around those figures."""
+    coords = total_c = weights = total_w = indices = 0
+    worst = 0.0
+    for seed in range(1, 6):
+        c = W.room_case(seed)
+        bm, s, parts = c["map"], c["step"], c["particles"]
+        rng = np.random.default_rng(500 + seed)
+        raw = W.sensor_cloud(bm, c["pose"], 100000, 0.05, rng)
+        clouds = [oracle.voxel_grid(raw, 0.1, mode) for mode in (0, 1)]
+        assert clouds[0].shape == clouds[1].shape
+        coords += int((clouds[0].view(np.int32) != clouds[1].view(np.int32)).sum())
+        total_c += clouds[0].size
+        _, prob = oracle.compute_grid(bm.points(), bm.min, bm.max, bm.res, c["sensor_dev"], exp_mode=1)
+        g = oracle.GridMeta(bm.min, bm.max, bm.res, prob)
+        out = []
+        for cloud in clouds:
+            op = oracle.pf_predict(parts, s.mods, s.deltas, s.z4)
+            op, _ = oracle.pf_update(op, g, cloud, s.ranges, s.alpha, s.sigma)
+            out.append((op[:, 4].astype(np.float64), oracle.resample_indices(op[:, 4], s.u01, 0)))
+        w0, w1 = out[0][0], out[1][0]
+        weights += int((w0 != w1).sum())
+        total_w += w0.size
+        nz = np.maximum(np.abs(w0), np.abs(w1)) > 0
+        if nz.any():
+            worst = max(worst, float((np.abs(w0 - w1)[nz] / np.maximum(np.abs(w0), np.abs(w1))[nz]).max()))
+        indices += int((out[0][1] != out[1][1]).sum())
+    print("voxel sum order: %d of %d centroid coordinates differ; %d of %d weights differ, worst relative %.2e; "
+          "%d resample indices differ" % (coords, total_c, weights, total_w, worst, indices))
+    assert coords > 0  # the two orders do differ
+    assert worst < 1e-3
+    assert indices <= 0.01 * total_w
