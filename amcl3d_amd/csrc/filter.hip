@@ -597,6 +597,7 @@ __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) 
 constexpr uint32_t kTiledMin = 16384;  // particles from which a single filter's sums run tiled
 constexpr uint32_t kWalkTiles = 1024;  // tile records the walk holds in LDS at a time
 constexpr uint32_t kSubTiles = kTiledTile / 64;
+static_assert(kSubTiles == 16, "a tile's sub-tiles fit one row of 16 lanes");
 
 struct TileRec {
   double dsum;          // dsum: the tile's terms summed in double
@@ -815,11 +816,14 @@ __device__ __forceinline__ float from_k(uint32_t k, int e, bool neg) {
 // and k stays below 2^24: every term in the binade), with kb and ka, k
 // before and after each lane's map.  A valid map adds less than 2^24, so 64
 // of them cannot wrap k.
+// ROW: the valid lanes are all in the first row of 16, whose composition
+// needs only the row's four DPP levels.
+template <bool ROW = false>
 __device__ __forceinline__ uint32_t run_maps(float s, int e, ParityFn m, bool valid, uint32_t& kb, uint32_t& ka) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t L = first_false(valid);
   if (!valid) m = ParityFn{0u, 0u, 0u, 1u};
-  const ParityFn inc = wave_inclusive_fn(m);
+  const ParityFn inc = ROW ? row_inclusive_fn(m) : wave_inclusive_fn(m);
   const uint32_t kin = (uint32_t)(fabs((double)s) * ldexp(1.0, 23 - e));  // in [2^23, 2^24)
   ka = kin + ((kin & 1u) ? inc.d1 : inc.d0);
   kb = (uint32_t)__shfl_up((int)ka, 1);
@@ -883,7 +887,7 @@ __device__ float walk_subtiles(float s, uint32_t t, uint32_t n, const float* x, 
       const bool sign_ok = (s > 0.0f) ? !(fl & kNeg) : !(fl & kPos);
       const bool valid = have && (use_a || use_b) && sign_ok && m.d0 < (1u << 24) && m.d1 < (1u << 24);
       uint32_t kb, ka;
-      const uint32_t C = run_maps(s, E, m, valid, kb, ka);
+      const uint32_t C = run_maps<true>(s, E, m, valid, kb, ka);  // at most 16 sub-tiles: lanes 0..15
       if (C > 0) {
         const bool neg = s < 0.0f;
         if (prefix && lane < C) prefix[(t0 + Q * 64) / kGroup] = from_k(ka, E, neg);
