@@ -322,6 +322,32 @@ def map_file(amcl3d_amd, bm, sensor_dev=0.05, log=print):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def room_step(amcl3d_amd, W, grid, reps=50, warmup=5):
+    """The reference's CPU-runnable configuration on the device: the room map
+    (200 x 200 x 60 at 0.1 m), 1,000 particles, 2,000 points, 3 beacons; one
+    synchronous predict -> update -> resample with host inputs, as the
+    reference's node calls it.  Median and p99 of reps steps."""
+    c = W.room_case(1)
+    s = c["step"]
+    pf = amcl3d_amd.ParticleFilter()
+    times = []
+    for r in range(warmup + reps):
+        pf.set_particles(c["particles"])
+        amcl3d_amd.synchronize()
+        t0 = time.perf_counter()
+        pf.predict(*s.mods, *s.deltas, s.z4)
+        pf.update(grid, s.cloud, s.ranges, s.alpha, s.sigma)
+        pf.resample(s.u01)
+        pf.synchronize()
+        if r >= warmup:
+            times.append(time.perf_counter() - t0)
+    pf.close()
+    t = np.array(times) * 1e3
+    return {"workload": "room map 200x200x60 at 0.1 m, 1,000 particles, 2,000 points, 3 beacons: one synchronous "
+                        "predict -> update -> resample with host inputs", "reps": reps,
+            "p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99))}
+
+
 def node_step(amcl3d_amd, wl, n=100000, steps=20, warmup=3):
     """Latency of Node::pointcloudCallback (Node.cpp:112-191) through the
     library's node: the large filter's building map, trajectory, 20,000-point
@@ -373,7 +399,9 @@ def node_step(amcl3d_amd, wl, n=100000, steps=20, warmup=3):
 def extras(amcl3d_amd, W, plain_value=None, building=None, building_map=None, wl=None, log=print,
            valu_per_64_evals=None):
     bm, grid = room_grid(amcl3d_amd, W)
-    out = {"batched": batched(amcl3d_amd, W, bm, grid, map_name="the 200x200x60 room map")}
+    out = {"room_step": room_step(amcl3d_amd, W, grid)}
+    log("room step:", out["room_step"])
+    out["batched"] = batched(amcl3d_amd, W, bm, grid, map_name="the 200x200x60 room map")
     log("batched:", out["batched"]["batched_evals_per_s"], "evals/s")
     if building is not None:
         out["batched_large_map"] = batched(amcl3d_amd, W, building[0], building[1],

@@ -347,8 +347,10 @@ def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):
     predict and update for the first S particles with the full cloud; S is
     sized to about args.cpu_seconds.  legs: the room configuration's full
     predict -> update -> resample step (1,000 particles, 2,000 points, 3
-    beacons; median of 7) and the host computeGrid (exact nearest neighbour)
-    of that 200 x 200 x 60 map, both single-threaded like the reference."""
+    beacons; median of 7), the host computeGrid (exact nearest neighbour) of
+    that 200 x 200 x 60 map and a seeded sample of filters of the batched
+    workload's distribution on it, single-threaded like the reference; and the large-filter sample on every
+    host core (not the reference, which is serial)."""
     import oracle
 
     from amcl3d_amd import workloads as W
@@ -367,8 +369,18 @@ def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):
     t = run(100)
     s = int(min(args.particles, max(100, 100 * args.cpu_seconds / max(t, 1e-6))))
     t = run(s)
-    del g
     legs = {}
+    # the same sample on every host core this process may use (the oracle's
+    # particles split over threads): not the reference, which is serial
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    p = oracle.pf_predict(start_state[:s], mods, wl["deltas"][k], wl["z4"][k][:s])
+    t0 = time.perf_counter()
+    oracle.pf_update(p, g, cloud, ranges, alpha, sigma, nthreads=threads)
+    tm = time.perf_counter() - t0
+    legs["large_filter_all_cores"] = {"kind": "not the reference (the oracle's update split over threads)",
+                                      "threads": threads, "particles": s, "seconds": tm,
+                                      "evals_per_s": s * args.points / tm}
+    del g
     # the room configuration: map build, then one full step
     c = W.room_case(1)
     bm = c["map"]
@@ -397,6 +409,29 @@ def cpu_baseline(grid, wl, start_state, args, mods, alpha, sigma, Wm):
                               "beacons": int(st.ranges.shape[0]), "full_step_ms_median": 1e3 * float(np.median(full)),
                               "update_ms_median": 1e3 * float(np.median(upd)),
                               "update_evals_per_s": ev / float(np.median(upd)), "runs": len(full), "threads": 1}
+    # filters of the batched workload's distribution (N log-uniform 500..50k,
+    # M 1k..30k, 1..8 beacons) on the same room map: a seeded sample of 16,
+    # one full step each, single-threaded, until about 5 s of CPU time
+    from amcl3d_amd import benchlib
+
+    counts, fparts, fclouds, franges = benchlib.fleet_case(W, bm, 16, 29)
+    rng = np.random.default_rng(12)
+    evals, secs, done = 0, 0.0, 0
+    for f in range(16):
+        pf_, cl, rg = fparts[f], fclouds[f], franges[f]
+        z4 = rng.standard_normal((pf_.shape[0], 4)).astype(np.float32)
+        t0 = time.perf_counter()
+        op = oracle.pf_predict(pf_, st.mods, st.deltas, z4)
+        op, _ = oracle.pf_update(op, og, cl, rg, 0.8, 0.1, nthreads=1)
+        oracle.pf_resample(op, 0.5)
+        secs += time.perf_counter() - t0
+        inside = np.all((op[:, :3].astype(np.float64) >= bm.min) & (op[:, :3].astype(np.float64) < bm.max), 1)
+        evals += int(inside.sum()) * cl.shape[0]
+        done += 1
+        if secs > 5.0:
+            break
+    legs["batched_filters_sample"] = {"filters": done, "seed": 29, "evals": evals, "seconds": secs,
+                                      "evals_per_s": evals / secs, "threads": 1}
     return {"value": s * args.points / t, "unit": "evals/s", "cores": 1, "kind": "port",
             "sample": "oracle ParticleFilter::update of the first %d of %d particles x %d points, one step, "
                       "single thread, %.1f s" % (s, args.particles, args.points, t),
