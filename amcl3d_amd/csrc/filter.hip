@@ -577,18 +577,20 @@ __global__ __launch_bounds__(kChainThreads) void mean_prefix_kernel(Filters fs) 
 // map (parity in -> increment, parity out) that depends on the terms and u
 // only (the map of exact_sum, composed over the run).  So:
 //   dsum  one workgroup per tile: the tile's terms summed in double, and
-//         their signs;
+//         their signs (or the kernel that writes the terms does it);
 //   map   one workgroup per tile: from the double sums of the earlier terms,
 //         guesses the binade s is in where the tile starts, and where each
 //         sub-tile starts; composes the tile's map under the tile's guess,
 //         and each sub-tile's maps under its guess and the binade above;
-//   walk  one workgroup per sum, from s = +0.  Wavefront 0 takes each tile
-//         whose map applies (s in the guessed binade, every term of the
-//         sign of s, the tile ending inside the binade) in O(1); any other
-//         tile sub-tile by sub-tile: by a sub-tile map when one applies,
-//         else as 64 float adds in order; tiles with terms of both signs go
-//         to exact_sum_range, the one-CU algorithm, run by the whole
-//         workgroup from the exact s;
+//   walk  one workgroup per sum, from s = +0.  Wavefront 0 takes a run of
+//         up to 64 tiles whose maps apply (s in the guessed binade, every
+//         term of the sign of s, k below 2^24 at the run's end) in one
+//         wavefront scan; a tile it cannot take whole, sub-tile by
+//         sub-tile: runs of sub-tile maps the same way, any other sub-tile
+//         as 64 float adds in order; a tile with terms of both signs, or of
+//         the other sign than s, and a sum that is not finite go to
+//         exact_sum_range, the one-CU algorithm, run by the whole workgroup
+//         from the exact s;
 //   ckpt  for resample's running sum c: the checkpoints (c after every 64th
 //         weight) of the tiles the walk took whole, from the exact s it
 //         recorded at their start (the walk writes the others').
