@@ -292,6 +292,7 @@ int a3d_fleet_init(a3d_fleet* fl, const float* poses, const float* devs, const f
   if (!fl || !poses || !devs || !z4) return fail(A3D_ERR_ARG, "a3d_fleet_init: null argument");
   if (int rc = refuse_capture(fl, "a3d_fleet_init")) return rc;
   fl->layout = false;  // the descriptors below replace the layout's
+  fl->staged = false;  // and the staged update's
   int rc = grow(fl->stage, fl->stage_cap, 4 * fl->total, fl->stream);
   if (rc) return rc;
   A3D_HIP(hipMemcpyAsync(fl->stage, z4, 4 * fl->total * 4, hipMemcpyHostToDevice, fl->stream));
@@ -321,6 +322,7 @@ int a3d_fleet_predict(a3d_fleet* fl, const double* mods, const double* deltas, c
   if (!fl || !mods || !deltas || !z4) return fail(A3D_ERR_ARG, "a3d_fleet_predict: null argument");
   if (int rc = refuse_capture(fl, "a3d_fleet_predict")) return rc;
   fl->layout = false;  // the descriptors below replace the layout's
+  fl->staged = false;  // and the staged update's
   int rc = grow(fl->stage, fl->stage_cap, 4 * fl->total, fl->stream);
   if (rc) return rc;
   A3D_HIP(hipMemcpyAsync(fl->stage, z4, 4 * fl->total * 4, hipMemcpyHostToDevice, fl->stream));

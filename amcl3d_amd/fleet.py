@@ -78,6 +78,11 @@ class Fleet:
     def resample(self, u01, mode=_lib.RESAMPLE_REFERENCE):
         _lib.call("a3d_fleet_resample", self._h, _lib.f32(u01), int(mode))
 
+    def predict(self, mods, deltas, z4):
+        """mods, deltas: (F, 4) per filter; z4: (total, 4) normals, filter by filter."""
+        _lib.call("a3d_fleet_predict", self._h, _lib.f64(np.ascontiguousarray(mods).reshape(-1)),
+                  _lib.f64(np.ascontiguousarray(deltas).reshape(-1)), _lib.f32(np.ascontiguousarray(z4).reshape(-1)))
+
     # device-parameter forms (capturable): see a3d_fleet_set_layout
     def set_layout(self, m, nb):
         self.layout_m = _lib.u32(m)

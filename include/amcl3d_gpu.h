@@ -268,7 +268,11 @@ int a3d_fleet_update(a3d_fleet* fl, const a3d_grid* g, const float* x, const flo
 int a3d_fleet_resample(a3d_fleet* fl, const float* u01, int mode);
 int a3d_fleet_get_means(const a3d_fleet* fl, float* means4);
 /* Same update with clouds, ranges and parameters already on the device
- * (uploaded once with a3d_fleet_stage_update); runs asynchronously. */
+ * (uploaded once with a3d_fleet_stage_update); runs asynchronously.  A
+ * staged update stays valid across updates and resamples; a3d_fleet_init,
+ * a3d_fleet_predict and a3d_fleet_set_layout replace the descriptors it
+ * staged, so stage again after them (a3d_fleet_update_staged refuses until
+ * then). */
 int a3d_fleet_stage_update(a3d_fleet* fl, const float* x, const float* y, const float* z, const uint32_t* m,
                            const float* ranges, const uint32_t* nb, const double* alpha, const double* sigma,
                            const double* roll, const double* pitch);

@@ -212,3 +212,66 @@ def test_fleet_dev_forms_refuse_without_layout(gpu):
         g.close()
     finally:
         fl.close()
+
+
+def test_fleet_host_predict_and_resample_match_oracle(gpu):
+    """The host forms a3d_fleet_predict and a3d_fleet_resample (through
+    amcl3d_amd.fleet.Fleet) on uneven filters: each filter bit for bit as the
+    oracle's predict and resample of that filter alone."""
+    from amcl3d_amd import fleet as fleet_mod
+
+    rng = np.random.default_rng(8)
+    F = 9
+    counts = np.round(np.exp(rng.uniform(np.log(5), np.log(4000), F))).astype(np.uint32)
+    parts = [W.particles_around(np.array([5.0 + f, 4.0, 1.5, rng.uniform(-3, 3)]), int(counts[f]), rng) for f in range(F)]
+    for p in parts:
+        p[:, 4] = rng.exponential(1.0, p.shape[0]).astype(F32)
+    mods = rng.uniform(0.05, 0.3, (F, 4))
+    deltas = rng.uniform(-0.2, 0.2, (F, 4))
+    z4 = rng.standard_normal((int(counts.sum()), 4)).astype(F32)
+    u01 = rng.uniform(0, 1, F).astype(F32)
+    fl = fleet_mod.Fleet(counts)
+    try:
+        fl.set_particles(np.concatenate(parts))
+        fl.predict(mods, deltas, z4)
+        got = fl.particles()
+        fl.resample(u01)
+        got_rs = fl.particles()
+    finally:
+        fl.close()
+    off = 0
+    for f in range(F):
+        n = int(counts[f])
+        op = oracle.pf_predict(parts[f], mods[f], deltas[f], z4[off:off + n])
+        assert np.array_equal(got[off:off + n].view(np.int32), op.view(np.int32)), f
+        oq, _ = oracle.pf_resample(op, float(u01[f]), 0)
+        assert np.array_equal(got_rs[off:off + n].view(np.int32), oq.view(np.int32)), f
+        off += n
+
+
+def test_fleet_predict_invalidates_the_staged_update(gpu):
+    """a3d_fleet_predict replaces the descriptors a staged update uploaded:
+    update_staged refuses until the update is staged again."""
+    from amcl3d_amd import fleet as fleet_mod
+
+    c = W.room_case(2)
+    bm = c["map"]
+    grid = gpu.Grid3d()
+    grid.open(bm.points(), bm.min, bm.max, bm.res, c["sensor_dev"])
+    counts = np.array([50, 70], np.uint32)
+    rng = np.random.default_rng(1)
+    fl = fleet_mod.Fleet(counts)
+    try:
+        fl.set_particles(np.concatenate([W.particles_around(c["pose"], int(n), rng) for n in counts]))
+        clouds = [c["step"].cloud[:100], c["step"].cloud[100:180]]
+        ranges = [c["step"].ranges, c["step"].ranges[:1]]
+        fl.stage_update(clouds, ranges, 0.8, 0.1)
+        fl.update_staged(grid)
+        fl.predict(np.full((2, 4), 0.1), np.full((2, 4), 0.05), rng.standard_normal((120, 4)).astype(F32))
+        with pytest.raises(_lib.Amcl3dError, match="stage_update"):
+            fl.update_staged(grid)
+        fl.stage_update(clouds, ranges, 0.8, 0.1)
+        fl.update_staged(grid)
+        fl.synchronize()
+    finally:
+        fl.close()
