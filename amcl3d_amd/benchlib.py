@@ -480,6 +480,9 @@ def targets(out, valu_per_64_evals):
          "valu_instr_per_64_evals": valu_per_64_evals}
     if "batched_large_map" in out:
         t["batched_over_plain_loop"]["building_map"] = out["batched_large_map"]["batched_over_plain_loop"]
+        if "converged" in out["batched_large_map"]:
+            t["batched_over_plain_loop"]["building_map_converged"] = \
+                out["batched_large_map"]["converged"]["batched_over_plain_loop"]
     if ceiling:
         # the plain loop already runs the large filters, most of the work, near
         # the issue rate: no batching can beat it by more than this
