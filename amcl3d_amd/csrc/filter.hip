@@ -1322,11 +1322,22 @@ __global__ void resample_select_kernel(Filters fs, int mode) {
       float c = gi ? cp[gi - 1] : 0.0f;  // 0 + w_0 is w_0 exactly
       const uint32_t e = min(n, (gi + 1) * kGroup);
       i = e - 1;
-      for (uint32_t k = gi * kGroup; k < e; ++k) {
-        c += f.w[k];
-        if (!(u > c)) {
-          i = k;
-          break;
+      // 16 weights loaded at a time (independent loads, one wait), then
+      // added in order until the first c that u does not exceed
+      bool found = false;
+      for (uint32_t k0 = gi * kGroup; k0 < e && !found; k0 += 16) {
+        float wv[16];
+#pragma unroll
+        for (uint32_t j = 0; j < 16; ++j) wv[j] = k0 + j < e ? f.w[k0 + j] : 0.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < 16; ++j) {
+          if (!found && k0 + j < e) {
+            c += wv[j];
+            if (!(u > c)) {
+              i = k0 + j;
+              found = true;
+            }
+          }
         }
       }
     }

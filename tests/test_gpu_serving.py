@@ -33,11 +33,12 @@ def _sessions(bm, k, seed):
     return out
 
 
-def _run(gpu, grid, sessions, alone, maps_of=None):
+def _run(gpu, grid, sessions, alone, maps_of=None, max_batch_evals=None):
     """grid: one grid, or a list of grids with maps_of[i] the map of session i."""
     from amcl3d_amd.serving import Server
 
-    srv = Server(max_inflight=1 if alone else 3, max_batch_sessions=1 if alone else 64)
+    kw = {} if max_batch_evals is None else {"max_batch_evals": max_batch_evals}
+    srv = Server(max_inflight=1 if alone else 3, max_batch_sessions=1 if alone else 64, **kw)
     grids = grid if isinstance(grid, list) else [grid]
     mids = [srv.add_map(g) for g in grids]
     maps_of = maps_of or [0] * len(sessions)
@@ -70,6 +71,22 @@ def test_same_alone_and_in_the_mix(gpu, room_grid):
     ma = {(r.session, r.ticket % 1000000): np.array(r.mean) for r in res_a}
     assert all(np.isfinite(r.latency_ms) and r.latency_ms >= 0 for r in res_m)
     assert len(ma) == 30
+
+
+def test_backlog_grows_batches_same_results(gpu, room_grid):
+    """A bound of about one session's evaluations per batch: with every
+    session's request queued at once the backlog lets batches grow past it
+    (fewer batches than requests), and every session still gives its alone
+    result."""
+    bm, grid = room_grid
+    sessions = _sessions(bm, 12, 31)
+    bound = int(np.median([n * len(st[0][0][0]) for n, _, _, st in sessions]))
+    alone, _, _ = _run(gpu, grid, sessions, alone=True)
+    grown, res, st = _run(gpu, grid, sessions, alone=False, max_batch_evals=bound)
+    assert len(res) == 36
+    assert st["batches"] < 36
+    for a, b in zip(alone, grown):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
 
 
 def test_served_equals_plain_filter(gpu, room_grid):
