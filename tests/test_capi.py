@@ -95,3 +95,29 @@ def test_cpp_layer_soft_failures_without_a_map():
     r = subprocess.run([shim, "unopened"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert '"weight": 0' in r.stdout and '"into_map": 0' in r.stdout
+
+
+def test_kernel_shapes_round_trip_and_refuse_bad_values():
+    """Every shape the autotuner tries (mapping, particles per wavefront,
+    order, cloud tile, XCD dealing) is set and read back as given, without a
+    GPU; out-of-range values are refused with a message and change nothing."""
+    from amcl3d_amd import autotune
+
+    L = _lib.load()
+    before = autotune.get_shape()
+    try:
+        names = set()
+        for sh in autotune.FLEET_CANDIDATES:
+            autotune.set_shape(sh)
+            assert autotune.get_shape() == sh
+            names.add(autotune.shape_name(sh))
+        assert len(names) == len(autotune.FLEET_CANDIDATES)  # names tell shapes apart
+        assert autotune.shape_name((0, 16, 1, 256, 4)) == "map0-ppw16-spatial-tile256-deal4"
+        autotune.set_shape((0, 16, 1, 128, 0))
+        for call, bad in ((L.a3d_set_cloud_tile, 48), (L.a3d_set_cloud_tile, 512), (L.a3d_set_deal, -1),
+                          (L.a3d_set_deal, 5000), (L.a3d_set_tile, 8), (L.a3d_set_mapping, 2)):
+            assert call(bad) == _lib.A3D_ERR_ARG
+            assert L.a3d_last_error()
+        assert autotune.get_shape() == (0, 16, 1, 128, 0)
+    finally:
+        autotune.set_shape(before)
