@@ -101,6 +101,7 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3, map_name="the
     that many update -> resample -> predict cycles (odometry 2 cm and 0.5 deg
     a cycle with the launch file's noise factors), after which the filters
     track their poses, and the same two timings again ("converged")."""
+    entry_shape = autotune.get_shape()  # restored on return: later phases run at the shape they found
     counts, parts, clouds, ranges = fleet_case(W, bm, nfilters, seed)
     fl = fleet_mod.Fleet(counts)
     fl.set_particles(np.concatenate(parts))
@@ -148,6 +149,7 @@ def batched(amcl3d_amd, W, bm, grid, nfilters=256, seed=3, reps=3, map_name="the
         conv = {"cycles": converge, "batched_evals_per_s": ev / tb, "plain_loop_evals_per_s": ev / tl,
                 "batched_over_plain_loop": tl / tb, "batched_update_ms": tb * 1e3, "plain_loop_update_ms": tl * 1e3}
     fl.close()
+    autotune.set_shape(entry_shape)
     out = {"workload": "%d filters on %s, N log-uniform 500..50k, M 1k..30k, 1..8 beacons" % (nfilters, map_name),
             "evals_per_update": ev, "shape": autotune.shape_name(shape),
             "autotune_s": {autotune.shape_name(k): v for k, v in timings.items()},
@@ -215,6 +217,7 @@ def served(amcl3d_amd, W, maps, nsessions=256, rate_hz=10.0, duration_s=8.0, see
                            % (nsessions, len(maps), rate_hz, n_range[0], n_range[1], m_range[0], m_range[1],
                               duration_s, latest_only, max_inflight, max_batch_evals))
         out["session_setup_s"] = setup_s
+        out["shape"] = autotune.shape_name(autotune.get_shape())
         if slo_ms:
             keep = min(1.0, out["served_evals_per_s"] / max(out["offered_evals_per_s"], 1.0))
             out["ladder"] = []

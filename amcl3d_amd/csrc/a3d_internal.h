@@ -65,6 +65,15 @@ struct GridDev {
   // and twice that.
   double near_w[3];
   double near_w2[3];
+  // 30 when cell indices may be formed by the 24-bit multiply-adds (sx <
+  // 2^24 and sy * sz < 2^24; the index then comes out modulo 2^32): a
+  // wavefront whose cells all lie in one 2^30-cell segment k may load them
+  // through a structured buffer based at prob + k * 2^30, whose 32-bit byte
+  // offset (record * 4 modulo 2^32) is then the offset inside the segment.
+  // 0: never.  The allocation of such a grid of more than 2^30 cells is
+  // rounded up to whole segments, so every such load stays inside it.
+  uint32_t seg_bits;
+  uint32_t pad2_;
 };
 
 // One filter and the parameters of the call being made on it.  Arrays are
@@ -78,6 +87,11 @@ struct FilterDev {
   float* wp;
   float* wr;
   float* scratch;  // scratch_floats(n): tail sums, resample prefix, gather staging
+  // 2 * kZrParts floats: the range of the finite rotated z of each of
+  // kZrParts parts of the cloud, written before the cloud weight
+  // (launch_weight) for its per-wavefront segment test; null: no segment
+  // loads
+  float* zr;
   uint8_t* inmap;  // n
   float* mean;     // 4
   const GridDev* gdev;  // device copy of this filter's grid; null: the launch's grid
@@ -162,10 +176,11 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g);
 // over a work list uses d_counter[0..7] (one queue counter per XCD).
 void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items, uint32_t nitems,
                    uint32_t* d_counter, int ppw, int persistent, hipStream_t s);
-// Poses given explicitly (a3d_grid_cloud_weight): out[i].
+// Poses given explicitly (a3d_grid_cloud_weight): out[i]; zr: 2 * kZrParts
+// floats of device memory for the cloud's z ranges.
 void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, const float* cz, uint32_t m,
                          const float* tx, const float* ty, const float* tz, const float* yaw, uint32_t n, float roll,
-                         float pitch, float* out, int ppw, hipStream_t s);
+                         float pitch, float* out, float* zr, int ppw, hipStream_t s);
 // Normalisation, blend and mean of ParticleFilter::update (ParticleFilter.cpp:150-195).
 void launch_update_tail(const GridDev& g, const Filters& fs, hipStream_t s);
 void launch_predict(const Filters& fs, hipStream_t s);
@@ -226,9 +241,10 @@ int current_order();    // A3D_ORDER_INDEX or A3D_ORDER_SPATIAL
 //   [12n, 13n) resample float running sum c: its value at the end of every
 //              group of 64 weights (ceil(n / 64) floats)
 //   then FP64 block totals, then 32 floats of per-filter sums (wtp, wtr, wt,
-//   c, the irregular-weights flag), then the tile records of the tiled exact
-//   sums (kTiledSums sums of tiled_tiles(n) records of 16 floats, then their
-//   sub-tile records, 16 per tile of 8 floats each)
+//   c, the irregular-weights flag; from kSumZr the cloud's z ranges, zr), then
+//   the tile records of the tiled exact sums (kTiledSums sums of
+//   tiled_tiles(n) records of 16 floats, then their sub-tile records, 16 per
+//   tile of 8 floats each)
 __host__ __device__ constexpr size_t sc_c64(uint32_t n) { return 10 * (size_t)n; }
 __host__ __device__ constexpr size_t sc_c32(uint32_t n) { return 12 * (size_t)n; }
 __host__ __device__ constexpr size_t sc_tot(uint32_t n) { return (13 * (size_t)n + 1) & ~(size_t)1; }
@@ -237,6 +253,9 @@ constexpr uint32_t kTiledTile = 1024;  // terms per tile of a tiled exact sum
 constexpr uint32_t kTiledSums = 5;     // sums one tiled launch computes at most
 __host__ __device__ constexpr size_t tiled_tiles(uint32_t n) { return ((size_t)n + kTiledTile - 1) / kTiledTile; }
 __host__ __device__ constexpr size_t sc_tiles(uint32_t n) { return (sc_sums(n) + 32 + 7) & ~(size_t)7; }
+constexpr uint32_t kSumZr = 8;    // per-filter sums slots kSumZr .. kSumZr + 2 * kZrParts - 1: FilterDev::zr
+constexpr uint32_t kZrParts = 12;  // parts of a cloud whose z ranges are taken in parallel
+static_assert(kSumZr + 2 * kZrParts <= 32, "the z ranges must fit the 32 per-filter sums");
 __host__ __device__ constexpr size_t scratch_floats(uint32_t n) {
   return sc_tiles(n) + (16 + 16 * 8) * kTiledSums * tiled_tiles(n);
 }

@@ -108,18 +108,47 @@ void grid_meta(const double mn[3], const double mx[3], double res, GridDev& g) {
   // needs sx < 2^24 and sy * sz < 2^24, and the zero cell after the last
   // (index cells) must fit a 32-bit record count; other grids take the
   // 64-bit path.
-  g.idx64 = (g.cells >= 0xffffffffull || g.size[0] >= (1u << 24) ||
-             (uint64_t)g.size[1] * g.size[2] >= (1ull << 24)) ? 1u : 0u;
-  // Test knob: A3D_FORCE_IDX64=1 takes the 64-bit index path on any map.
+  const bool mad24 = g.size[0] < (1u << 24) && (uint64_t)g.size[1] * g.size[2] < (1ull << 24);
+  g.idx64 = (g.cells >= 0xffffffffull || !mad24) ? 1u : 0u;
+  // A structured buffer forms record * 4 in 32 bits (modulo 2^32): a 2^30-cell
+  // segment of any grid whose indices the multiply-adds form is one buffer.
+  g.seg_bits = mad24 ? 30u : 0u;
+  // Test knobs: A3D_FORCE_IDX64=1 takes the 64-bit index path on any map
+  // (=2: the same kernels, with their per-wavefront segment loads).
   const char* force = std::getenv("A3D_FORCE_IDX64");
-  if (force && force[0] == '1') g.idx64 = 1u;
-  // A structured buffer forms record * 4 in 32 bits: only a prob array whose
-  // bytes, the zero cell included, fit 2^32 may be read through one.
+  if (force && (force[0] == '1' || force[0] == '2')) {
+    g.idx64 = 1u;
+    if (force[0] == '1') g.seg_bits = 0u;
+  }
+  // A structured buffer based at prob: only a prob array whose bytes, the
+  // zero cell included, fit 2^32 may be read through one.
   g.buf32 = (!g.idx64 && (g.cells + 1) * 4 <= 0x100000000ull) ? 1u : 0u;
   // Test knob: A3D_FORCE_IDX32=1 reads 32-bit indices through 64-bit
-  // addresses on any map that has 32-bit indices.
+  // addresses on any map that has 32-bit indices (=2: the same kernels, with
+  // their per-wavefront segment loads).
   const char* force32 = std::getenv("A3D_FORCE_IDX32");
-  if (force32 && force32[0] == '1') g.buf32 = 0u;
+  if (force32 && (force32[0] == '1' || force32[0] == '2')) {
+    g.buf32 = 0u;
+    if (force32[0] == '1') g.seg_bits = 0u;
+  }
+  // Test knob: A3D_SEG_TEST_BITS=b (8..29) lets a wavefront take the segment
+  // loads only when its cells lie in one 2^b-cell block, so that small maps
+  // mix wavefronts of both forms; b = 0: no segment loads.
+  const char* segb = std::getenv("A3D_SEG_TEST_BITS");
+  if (segb && g.seg_bits) {
+    const int b = std::atoi(segb);
+    if (b == 0 || (b >= 8 && b < 30)) g.seg_bits = (uint32_t)b;
+  }
+}
+
+// Bytes of a grid's prob allocation: the cells and the zero cell, rounded up
+// to whole 2^30-cell segments when segment loads may read it (a segment's
+// buffer then never reaches past the allocation).
+inline size_t prob_alloc_bytes(const GridDev& g) {
+  const uint64_t b = (g.cells + 1) * sizeof(float);
+  const uint64_t seg = 1ull << 32;
+  if (g.seg_bits && g.cells + 1 > (1ull << 30)) return (size_t)((b + seg - 1) / seg * seg);
+  return (size_t)b;
 }
 
 }  // namespace a3d
@@ -153,6 +182,7 @@ struct a3d_pf {
     f.wp = x + 5 * (size_t)n;
     f.wr = x + 6 * (size_t)n;
     f.scratch = scratch;
+    f.zr = scratch ? scratch + sc_sums(n) + kSumZr : nullptr;
     f.inmap = inmap;
     f.mean = mean;
     f.n = n;
@@ -359,7 +389,7 @@ static int grid_alloc(const double mn[3], const double mx[3], double res, double
   G->sensor_dev = sensor_dev;
   const size_t bytes = G->g.cells * sizeof(float);
   // one more float, 0, after the last cell: the cell that rejected points read
-  if (hipMalloc(&G->prob, bytes + sizeof(float)) != hipSuccess) {
+  if (hipMalloc(&G->prob, prob_alloc_bytes(G->g)) != hipSuccess) {
     delete G;
     return fail(A3D_ERR_NOMEM, "grid: cannot allocate " + std::to_string(bytes) + " bytes of prob");
   }
@@ -496,6 +526,7 @@ int a3d_grid_cloud_weight(const a3d_grid* g, const float* x, const float* y, con
   float* dtz = dty + n;
   float* dya = dtz + n;
   float* dout = dya + n;
+  float* dzr = dout + n;  // 2 * kZrParts of the 64 spare floats
   int rc = A3D_OK;
   hipStream_t s = g->stream;
   if (m) {
@@ -510,7 +541,8 @@ int a3d_grid_cloud_weight(const a3d_grid* g, const float* x, const float* y, con
               !hip_ok(hipMemcpyAsync(dya, yaw, n * 4, hipMemcpyHostToDevice, s), "upload")))
     rc = A3D_ERR_HIP;
   if (!rc) {
-    launch_weight_poses(g->g, cx, cy, cz, (uint32_t)m, dtx, dty, dtz, dya, (uint32_t)n, roll, pitch, dout, g_ppw, s);
+    launch_weight_poses(g->g, cx, cy, cz, (uint32_t)m, dtx, dty, dtz, dya, (uint32_t)n, roll, pitch, dout, dzr, g_ppw,
+                        s);
     if (!hip_ok(hipGetLastError(), "cloud weight kernel") ||
         !hip_ok(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, s), "download") ||
         !hip_ok(hipStreamSynchronize(s), "cloud weight"))

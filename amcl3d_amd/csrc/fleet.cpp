@@ -73,6 +73,7 @@ struct a3d_fleet {
     d.wp = state + 5 * total + o;
     d.wr = state + 6 * total + o;
     d.scratch = scratch + soff[f];
+    d.zr = scratch ? d.scratch + sc_sums(n[f]) + kSumZr : nullptr;
     d.inmap = inmap + o;
     d.mean = mean + 4 * (size_t)f;
     d.n = n[f];

@@ -79,6 +79,7 @@ extern "C" int a3d_pf_update_noalloc(const a3d_grid* g, float* d_particles, size
   f.wp = d_particles + 5 * n;
   f.wr = d_particles + 6 * n;
   f.scratch = reinterpret_cast<float*>(base);
+  f.zr = f.scratch + sc_sums(N) + kSumZr;
   f.inmap = base + kSumsBytes;
   f.mean = d_mean;
   f.n = N;

@@ -240,8 +240,18 @@ constexpr int kBufWord3 = 0x00020000;
 //            of 2^30 to 2^32 - 2 cells.
 //   kIdx64   64-bit cell indices (GridDev::idx64).
 // Index `cells` is the zero cell.  Code for a wider kind reads every grid a
-// narrower kind can (a batch over several maps runs the widest).
-enum IndexKind { kIdxBuf = 0, kIdx32 = 1, kIdx64 = 2 };
+// narrower kind can (a batch over several maps runs the widest).  Kernels of
+// the two wider kinds let a wavefront whose cells all lie in one 2^30-cell
+// segment k (wave_segment) read them as
+//   kIdxSeg  32-bit cell indices (modulo 2^32) through a structured buffer
+//            based at prob + k * 2^30: the hardware's 32-bit byte offset
+//            (index * 4 modulo 2^32) is the cell's offset inside the segment.
+//            A point outside the map reads record 0xffffffff, past the
+//            buffer's 0xfffffffe records, which loads 0 without a memory
+//            access (adding 0 leaves every in-order sum unchanged: the sum
+//            starts at +0 and a round-to-nearest sum is -0 only when both
+//            addends are).
+enum IndexKind { kIdxBuf = 0, kIdx32 = 1, kIdx64 = 2, kIdxSeg = 3 };
 
 template <int IK>
 struct ProbSrc;
@@ -251,7 +261,7 @@ struct ProbSrc<kIdxBuf> {
   using Idx = uint32_t;
   __amdgpu_buffer_rsrc_t r;
   Idx zero;
-  __device__ explicit ProbSrc(const GridDev& g)
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0)
       : r(__builtin_amdgcn_make_buffer_rsrc((void*)g.prob, (short)4, (int)(uint32_t)(g.cells + 1), kBufWord3)),
         zero((uint32_t)g.cells) {}
   __device__ float operator()(Idx i) const { return a3d_struct_load_f32(r, (int)i, 0, 0, 0); }
@@ -262,7 +272,7 @@ struct ProbSrc<kIdx32> {
   using Idx = uint32_t;
   const float* p;
   Idx zero;
-  __device__ explicit ProbSrc(const GridDev& g) : p(g.prob), zero((uint32_t)g.cells) {}
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p(g.prob), zero((uint32_t)g.cells) {}
   __device__ float operator()(Idx i) const { return p[(uint64_t)i]; }
 };
 
@@ -271,8 +281,22 @@ struct ProbSrc<kIdx64> {
   using Idx = uint64_t;
   const float* p;
   Idx zero;
-  __device__ explicit ProbSrc(const GridDev& g) : p(g.prob), zero(g.cells) {}
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p(g.prob), zero(g.cells) {}
   __device__ float operator()(Idx i) const { return p[i]; }
+};
+
+// k: the wavefront's segment (uniform).
+template <>
+struct ProbSrc<kIdxSeg> {
+  using Idx = uint32_t;
+  static constexpr uint32_t kRecords = 0xfffffffeu;
+  __amdgpu_buffer_rsrc_t r;
+  Idx zero;
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t k)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)(g.prob + ((uint64_t)k << 30)), (short)4, (int)kRecords,
+                                            kBufWord3)),
+        zero(0xffffffffu) {}
+  __device__ float operator()(Idx i) const { return a3d_struct_load_f32(r, (int)i, 0, 0, 0); }
 };
 
 // The pose of a lane's particle.  A lane without an active particle has
@@ -505,17 +529,19 @@ __device__ inline float4 cloud_point(const FilterDev& f, uint32_t j) {
 // filters).
 template <int Q, bool RP0, int IK, int MODE, bool BLOCK>
 __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f, const LanePose& lp, float& wsum,
-                                           int& cnt, float4* cl, bool run, uint32_t chunk) {
+                                           int& cnt, float4* cl, bool run, uint32_t chunk, uint32_t segk) {
   using Idx = typename ProbSrc<IK>::Idx;
-  const ProbSrc<IK> src(g);
+  const ProbSrc<IK> src(g, segk);
   const int lane = lane_id();
   const int po = lane % Q;
   const uint32_t m = f.m;
   // Each lane counts its own accepted points (an integer count is order-free;
   // the caller adds the Q lanes of a particle at the end).  The float sum is
   // not: the Q values of a step reach the particle's first lane and are added
-  // in point order.  A rejected point reads the zero cell, -0.0f, which leaves
-  // every sum unchanged (x + -0.0f == x, also for x = -0.0f).
+  // in point order.  A rejected point reads the zero cell (or, through a
+  // segment's buffer, an out-of-range record): +0, which leaves the sum
+  // unchanged, since the sum starts at +0 and a round-to-nearest sum is -0
+  // only when both addends are.
   auto accumulate = [&](float v) {
     if constexpr (Q == 1) {
       wsum += v;
@@ -587,6 +613,134 @@ __device__ __forceinline__ void lanes_loop(const GridDev& g, const FilterDev& f,
   }
 }
 
+// The loop variant of a wavefront: uniform per wavefront; in BLOCK form
+// wavefronts of one workgroup may take different ones, each passing the same
+// barriers.
+template <int Q, int IK, bool CLEAN, bool BLOCK>
+__device__ __forceinline__ void lanes_variant_loop(const GridDev& g, const FilterDev& f, const LanePose& lp,
+                                                   float& wsum, int& cnt, float4* cl, bool run, uint32_t chunk,
+                                                   uint32_t segk) {
+  if constexpr (CLEAN) {
+    const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;
+    if (f.rp0) {
+      if (zero)
+        lanes_loop<Q, true, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+      else
+        lanes_loop<Q, true, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+    } else {
+      if (zero)
+        lanes_loop<Q, false, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+      else
+        lanes_loop<Q, false, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+    }
+  } else {
+    if (f.rp0)
+      lanes_loop<Q, true, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+    else
+      lanes_loop<Q, false, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
+  }
+}
+
+// The range [lo, hi] of the finite rotated z of each of kZrParts parts of
+// each filter's cloud, into f.zr[2p], f.zr[2p + 1] (filters without zr are
+// skipped): a point's rotated z, sz = px*r20 + py*r21 + pz*r22 (pz itself
+// with roll = pitch = 0, as the weight kernels stage it), is the same float
+// under every particle of the filter, since the third row of the rotation
+// depends on roll and pitch only.  A non-finite sz fails the z test under
+// every pose and is left out (an empty part gives [inf, -inf]).  Grid
+// (kZrParts, filters).
+constexpr int kZrThreads = 256;
+__global__ __launch_bounds__(kZrThreads) void cloud_zrange_kernel(Filters fs) {
+  const FilterDev& f = fs.arr ? fs.arr[blockIdx.y] : fs.one;
+  if (!f.zr) return;  // uniform
+  float r[9];
+  rotation9(f.roll, f.pitch, 0.0f, r);
+  const float inf = __builtin_inff();
+  float lo = inf, hi = -inf;
+  const uint32_t m = f.m, part = (m + kZrParts - 1) / kZrParts;
+  const uint32_t j0 = blockIdx.x * part, j1 = min(m, j0 + part);
+  const bool rp0 = f.rp0 != 0;
+  constexpr int U = 8;  // loads in flight per thread
+  for (uint32_t jb = j0 + threadIdx.x; jb < j1; jb += U * kZrThreads) {
+    float z[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = jb + (uint32_t)u * kZrThreads;
+      z[u] = inf;
+      if (j < j1) z[u] = rp0 ? f.cz[j] : f.cx[j] * r[6] + f.cy[j] * r[7] + f.cz[j] * r[8];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (fabsf(z[u]) < inf) {
+        lo = fminf(lo, z[u]);
+        hi = fmaxf(hi, z[u]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    lo = fminf(lo, __shfl_xor(lo, o));
+    hi = fmaxf(hi, __shfl_xor(hi, o));
+  }
+  __shared__ float red[2 * (kZrThreads / 64)];
+  const uint32_t w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    red[2 * w] = lo;
+    red[2 * w + 1] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 0; v < kZrThreads / 64; ++v) {
+      lo = fminf(lo, red[2 * v]);
+      hi = fmaxf(hi, red[2 * v + 1]);
+    }
+    f.zr[2 * blockIdx.x] = lo;
+    f.zr[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+// Whether every cell this wavefront can gather over f's cloud lies in one
+// 2^30-cell segment (then *k is its number), for a grid with seg_bits and a
+// filter whose cloud z range [lo, hi] is in f.zr by parts (cloud_zrange_kernel).
+//   q = (float)((double)sz + oz) does not decrease in sz nor in oz, nor does
+//   the cell index floor(q / res) in q, so every in-map cell of the
+//   wavefront has iz within one of [iz(lo + oz_min), iz(hi + oz_max)] (one
+//   for the reciprocal product, which floors at most one apart from the
+//   division), clamped to the map: its index lies in [izl * sx * sy,
+//   (izh + 1) * sx * sy - 1].  An empty range (no finite point, no active
+//   lane, or every q outside the map) means no in-map cell: any segment
+//   serves.  The last check keeps the top index's low 32 bits below the
+//   buffer's record count.
+__device__ __forceinline__ bool wave_segment(const GridDev& g, const FilterDev& f, bool act, double oz,
+                                             uint32_t* k) {
+  float lo = f.zr[0], hi = f.zr[1];
+#pragma unroll
+  for (uint32_t p = 1; p < kZrParts; ++p) {
+    lo = fminf(lo, f.zr[2 * p]);
+    hi = fmaxf(hi, f.zr[2 * p + 1]);
+  }
+  const double inf = __builtin_inf();
+  double ol = act ? oz : inf, oh = act ? oz : -inf;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    ol = fmin(ol, __shfl_xor(ol, o));
+    oh = fmax(oh, __shfl_xor(oh, o));
+  }
+  const double qlo = (double)(float)((double)lo + ol), qhi = (double)(float)((double)hi + oh);
+  double zl = floor(qlo * g.inv_res) - 1.0, zh = floor(qhi * g.inv_res) + 1.0;
+  const double top = (double)g.size[2] - 1.0;
+  if (!(zl >= 0.0)) zl = 0.0;
+  if (!(zh <= top)) zh = top;
+  uint32_t seg = 1u, kk = 0u;
+  if (zl <= zh) {
+    const uint64_t a = (uint64_t)zl * g.step_z, b = ((uint64_t)zh + 1) * g.step_z - 1;
+    kk = (uint32_t)(a >> 30);
+    seg = ((a >> g.seg_bits) == (b >> g.seg_bits) && (uint32_t)b < ProbSrc<kIdxSeg>::kRecords) ? 1u : 0u;
+  }
+  *k = (uint32_t)__builtin_amdgcn_readfirstlane((int)kk);
+  return __builtin_amdgcn_readfirstlane((int)seg) != 0;
+}
+
 // CLEAN: the launch's grids are all clean (GridDev::clean).  BLOCK: the
 // workgroup walks the cloud together (every wavefront calls this, count may
 // be 0); cl then holds two chunks per workgroup, else one per wavefront.
@@ -620,26 +774,14 @@ __device__ void weight_lanes(const GridDev& g, const FilterDev& f, uint32_t p0, 
   if (KFILTER && act && po == 0) wr = range_weight(tx, ty, tz, f.ranges, f.nb, f.sigma);
   float wsum = 0.f;
   int cnt = 0;
-  // the branches are uniform per wavefront; in BLOCK form wavefronts of one
-  // workgroup may take different ones, each passing the same barriers
-  if constexpr (CLEAN) {
-    const bool zero = __builtin_amdgcn_ballot_w64(lp.slow) != 0;
-    if (f.rp0) {
-      if (zero)
-        lanes_loop<Q, true, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
-      else
-        lanes_loop<Q, true, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
-    } else {
-      if (zero)
-        lanes_loop<Q, false, IK, kCleanZero, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
-      else
-        lanes_loop<Q, false, IK, kClean, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
-    }
-  } else {
-    if (f.rp0)
-      lanes_loop<Q, true, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
+  if constexpr (IK != kIdxBuf) {
+    uint32_t segk = 0;
+    if (g.seg_bits && f.zr && wave_segment(g, f, act, lp.oz, &segk))
+      lanes_variant_loop<Q, kIdxSeg, CLEAN, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, segk);
     else
-      lanes_loop<Q, false, IK, kWindow, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk);
+      lanes_variant_loop<Q, IK, CLEAN, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, 0u);
+  } else {
+    lanes_variant_loop<Q, IK, CLEAN, BLOCK>(g, f, lp, wsum, cnt, cl, run, chunk, 0u);
   }
   if (Q > 1) {
     int c = cnt;
@@ -685,7 +827,8 @@ __global__ __launch_bounds__(kBlockThreads) void lanes_poses_kernel(GridDev g, F
 // a whole workgroup: wavefront w holds particles p0 + w*P .. and the
 // wavefronts walk the filter's cloud together.  A filter that names its own
 // grid (a batch over several maps) is weighted on that grid, else on the
-// launch's.
+// launch's; the launch grid's seg_bits = 0 keeps every filter off segment
+// loads.
 template <int Q, int IK, bool CLEAN>
 __device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f, const WorkItem& it, float4* cl,
                                            uint32_t chunk) {
@@ -693,15 +836,18 @@ __device__ __forceinline__ void lanes_item(const GridDev& g, const FilterDev& f,
   const uint32_t w = threadIdx.x >> 6, p0 = it.p0 + w * P;
   const uint32_t count = (w * P < it.count) ? min(P, it.count - w * P) : 0u;
   if (f.gdev) {
-    const GridDev gf = *f.gdev;
+    GridDev gf = *f.gdev;
+    if (!g.seg_bits) gf.seg_bits = 0u;  // the launch holds off segment loads
     weight_lanes<Q, IK, true, CLEAN, true>(gf, f, p0, count, nullptr, cl, chunk);
   } else {
     weight_lanes<Q, IK, true, CLEAN, true>(g, f, p0, count, nullptr, cl, chunk);
   }
 }
 
+// Clean grids: at most 128 registers, four wavefronts per SIMD (the loads
+// of spread particle sets need the wavefronts to hide their latency).
 template <int Q, int IK, bool CLEAN>
-__global__ __launch_bounds__(kBlockThreads) void lanes_fleet_kernel(GridDev g, const FilterDev* fs,
+__global__ __launch_bounds__(kBlockThreads, CLEAN ? 4 : 1) void lanes_fleet_kernel(GridDev g, const FilterDev* fs,
                                                                     const WorkItem* items, uint32_t nitems,
                                                                     uint32_t* counter, int persistent,
                                                                     uint32_t chunk, uint32_t deal) {
@@ -960,6 +1106,10 @@ void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items,
   ppw = clamp_ppw(ppw);
   if (current_mapping() == 0) {  // lanes = particles, ppw particles per wavefront
     const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
+    // the clouds' z ranges for the kernels' segment loads (a launch of a
+    // wider index kind that allows them)
+    if (index_kind(g) != kIdxBuf && g.seg_bits && fs.count > 0 && (fs.arr || fs.one.n > 0))
+      cloud_zrange_kernel<<<dim3(kZrParts, fs.count), kZrThreads, 0, s>>>(fs);
     if (!fs.arr) {
       if (fs.one.n == 0) return;
       if (ppw == 64) lanes_single_q<1>(g, fs.one, chunk, s);
@@ -988,9 +1138,10 @@ void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items,
 
 void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, const float* cz, uint32_t m,
                          const float* tx, const float* ty, const float* tz, const float* yaw, uint32_t n, float roll,
-                         float pitch, float* out, int ppw, hipStream_t s) {
+                         float pitch, float* out, float* zr, int ppw, hipStream_t s) {
   if (n == 0) return;
   FilterDev f{};
+  f.zr = zr;
   f.x = const_cast<float*>(tx);
   f.y = const_cast<float*>(ty);
   f.z = const_cast<float*>(tz);
@@ -1006,6 +1157,7 @@ void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, con
   ppw = clamp_ppw(ppw);
   if (current_mapping() == 0) {
     const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
+    if (index_kind(g) != kIdxBuf && g.seg_bits) cloud_zrange_kernel<<<dim3(kZrParts, 1), kZrThreads, 0, s>>>(single(f));
     if (ppw == 64) lanes_poses_q<1>(g, f, out, chunk, s);
     else if (ppw == 32) lanes_poses_q<2>(g, f, out, chunk, s);
     else lanes_poses_q<4>(g, f, out, chunk, s);

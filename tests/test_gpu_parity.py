@@ -254,6 +254,73 @@ def test_cloud_weights_32bit_index_64bit_address_path(gpu, room, shape, monkeypa
     assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
+def _segment_waves(bm, cloud, tz, per_wave, bits):
+    """Which wavefronts of the poses kernel (per_wave consecutive poses each)
+    read through segment loads under A3D_SEG_TEST_BITS=bits, with roll =
+    pitch = 0: the device's decision restated.  The cells of a wavefront lie
+    in layers [iz(lo + oz_min) - 1, iz(hi + oz_max) + 1] (lo, hi: the cloud's
+    finite z range; oz = tz - min z), and it takes the segment loads when
+    those layers' cells lie in one 2^bits-cell block."""
+    z = cloud[:, 2][np.isfinite(cloud[:, 2])].astype(np.float64)
+    lo, hi = z.min(), z.max()
+    size = [int(v) for v in bm.size]
+    layer = size[0] * size[1]
+    inv = 1.0 / bm.res
+    out = []
+    for w0 in range(0, tz.size, per_wave):
+        oz = tz[w0:w0 + per_wave].astype(np.float64) - np.float64(bm.min[2])
+        qlo = np.float64(np.float32(lo + oz.min()))
+        qhi = np.float64(np.float32(hi + oz.max()))
+        zl = max(np.floor(qlo * inv) - 1.0, 0.0)
+        zh = min(np.floor(qhi * inv) + 1.0, size[2] - 1.0)
+        if zl > zh:
+            out.append(True)
+            continue
+        a, b = int(zl) * layer, (int(zh) + 1) * layer - 1
+        out.append((a >> bits) == (b >> bits))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("bits", [19, 20])
+def test_cloud_weights_segment_loads_mixed_wavefronts(gpu, room, shape, monkeypatch, bits):
+    """Segment loads (a structured buffer based at the 2^30-cell segment that
+    holds all of a wavefront's cells) on the room map through the 32-bit
+    index kernels, with a test block of 2^bits cells so that some wavefronts
+    take them and the others the 64-bit addresses: bit-exact against the
+    oracle.  A band of the scan 0.5 m high and poses sorted by height keep
+    each wavefront's cells within a few layers."""
+    monkeypatch.setenv("A3D_FORCE_IDX32", "2")
+    monkeypatch.setenv("A3D_SEG_TEST_BITS", str(bits))
+    bm = room["map"]
+    g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
+    s = room["step"]
+    c = np.ascontiguousarray(s.cloud[(s.cloud[:, 2] > -0.3) & (s.cloud[:, 2] < 0.2)])
+    rng = np.random.default_rng(40 + bits)
+    p = W.particles_around(room["pose"], 1280, rng, dxyz=1.5, dyaw=3.0)
+    p = p[np.argsort(p[:, 2], kind="stable")]
+    w = g.cloud_weights(c, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    ow = oracle.cloud_weight_batch(room["ogrid"], c, p[:, 0], p[:, 1], p[:, 2], p[:, 3])
+    assert (ow > 0).mean() > 0.5
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+    if shape[0] == gpu.MAP_PARTICLE_LANES:
+        seg = _segment_waves(bm, c, p[:, 2], shape[1], bits)
+        _say("segment loads in %d of %d wavefronts" % (seg.sum(), seg.size))
+        assert seg.any() and not seg.all()
+
+
+def test_cloud_weights_64bit_index_segment_loads(gpu, room, shape, monkeypatch):
+    """The 64-bit index kernels with their segment loads (every wavefront of
+    the room map is in segment 0), roll and pitch included."""
+    monkeypatch.setenv("A3D_FORCE_IDX64", "2")
+    bm = room["map"]
+    g = gpu.Grid3d().from_cells(room["oprob"], bm.min, bm.max, bm.res, room["sensor_dev"])
+    s, p = room["step"], room["particles"]
+    for roll, pitch in ((0.0, 0.0), (0.05, -0.02)):
+        w = g.cloud_weights(s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        ow = oracle.cloud_weight_batch(room["ogrid"], s.cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
 def test_cloud_weights_window_test_on_clean_map(gpu, room, shape, monkeypatch):
     """The room's 0.1 m grid is clean (the reciprocal product floors like the
     division for every float), so it normally runs without the window test;
