@@ -267,21 +267,27 @@ struct ProbSrc<kIdxBuf> {
   __device__ float operator()(Idx i) const { return a3d_struct_load_f32(r, (int)i, 0, 0, 0); }
 };
 
+// prob as a global-memory pointer: a grid read through a descriptor in
+// memory (FilterDev::gdev) would otherwise be loaded with flat instructions,
+// whose outstanding loads also hold back every wait for the LDS reads of the
+// cloud tile.
+using GlobalFloat = const __attribute__((address_space(1))) float;
+
 template <>
 struct ProbSrc<kIdx32> {
   using Idx = uint32_t;
-  const float* p;
+  GlobalFloat* p;
   Idx zero;
-  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p(g.prob), zero((uint32_t)g.cells) {}
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p((GlobalFloat*)g.prob), zero((uint32_t)g.cells) {}
   __device__ float operator()(Idx i) const { return p[(uint64_t)i]; }
 };
 
 template <>
 struct ProbSrc<kIdx64> {
   using Idx = uint64_t;
-  const float* p;
+  GlobalFloat* p;
   Idx zero;
-  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p(g.prob), zero(g.cells) {}
+  __device__ explicit ProbSrc(const GridDev& g, uint32_t = 0) : p((GlobalFloat*)g.prob), zero(g.cells) {}
   __device__ float operator()(Idx i) const { return p[i]; }
 };
 
