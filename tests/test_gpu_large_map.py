@@ -109,6 +109,43 @@ def test_cloud_weights_upper_storeys(gpu, building, z):
     assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
+def test_cloud_weights_segment_boundary_roll_pitch(gpu, building):
+    """Poses around the first 4 GiB segment boundary (cell index 2^30, at
+    z = 13.4 m), a band of the scan 0.6 m high, poses sorted by height, with
+    and without roll and pitch: each wavefront bounds its cells through the
+    cloud's rotated z and reads them through its segment's buffer or,
+    straddling, through 64-bit addresses; bit-exact against the oracle
+    either way.  The oracle's cells show wavefronts of 16 poses wholly on
+    one side of the boundary and across it."""
+    bm, g, og = building["bm"], building["grid"], building["og"]
+    rng = np.random.default_rng(41)
+    pose, view = _pose(bm, 13.5, rng)
+    full = W.sensor_cloud(view, pose, 20000, 0.03, rng)
+    cloud = np.ascontiguousarray(full[np.abs(full[:, 2]) < 0.3])
+    p = W.particles_around(pose, 640, rng, dxyz=1.0, dyaw=0.5)
+    p = p[np.argsort(p[:, 2], kind="stable")]
+    for roll, pitch in ((0.0, 0.0), (0.04, -0.03)):
+        w = g.cloud_weights(cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        ow = oracle.cloud_weight_batch(og, cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        cells = oracle.cloud_cells(og, cloud, p[:, 0], p[:, 1], p[:, 2], p[:, 3], roll, pitch)
+        side = []
+        for w0 in range(0, p.shape[0], 16):
+            c = cells[w0:w0 + 16]
+            c = c[c >= 0]
+            if c.size:
+                side.append((int(c.min() >> 30), int(c.max() >> 30)))
+        below = sum(1 for lo, hi in side if hi == 0)
+        above = sum(1 for lo, hi in side if lo == 1)
+        across = sum(1 for lo, hi in side if lo == 0 and hi == 1)
+        _say("roll %.2f pitch %.2f: %d points; wavefronts of 16 below / above / across 2^30: %d / %d / %d"
+             % (roll, pitch, cloud.shape[0], below, above, across))
+        # level: wavefronts on both sides and across; tilted, the band's
+        # rotated z spans more layers, and more wavefronts straddle
+        assert across and (below or above) and (roll != 0.0 or (below and above))
+        assert (ow > 0).mean() > 0.9
+        assert np.array_equal(w.view(np.int32), ow.view(np.int32))
+
+
 def test_captured_step_top_storey(gpu, building):
     """One predict -> update -> resample step, captured in a HIP graph as the
     large filter runs it, with 5,000 particles on the top storey and 8

@@ -306,6 +306,11 @@ def test_cloud_weights_segment_loads_mixed_wavefronts(gpu, room, shape, monkeypa
         seg = _segment_waves(bm, c, p[:, 2], shape[1], bits)
         _say("segment loads in %d of %d wavefronts" % (seg.sum(), seg.size))
         assert seg.any() and not seg.all()
+    # with roll and pitch the rotated z mixes x and y in (the range the
+    # wavefronts are bounded by is that of px*r20 + py*r21 + pz*r22)
+    w = g.cloud_weights(c, p[:, 0], p[:, 1], p[:, 2], p[:, 3], 0.05, -0.03)
+    ow = oracle.cloud_weight_batch(room["ogrid"], c, p[:, 0], p[:, 1], p[:, 2], p[:, 3], 0.05, -0.03)
+    assert np.array_equal(w.view(np.int32), ow.view(np.int32))
 
 
 def test_cloud_weights_64bit_index_segment_loads(gpu, room, shape, monkeypatch):
