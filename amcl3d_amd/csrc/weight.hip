@@ -654,10 +654,10 @@ __device__ __forceinline__ void lanes_variant_loop(const GridDev& g, const Filte
 // under every particle of the filter, since the third row of the rotation
 // depends on roll and pitch only.  A non-finite sz fails the z test under
 // every pose and is left out (an empty part gives [inf, -inf]).  Grid
-// (kZrParts, filters).
+// (kZrParts, filters from base).
 constexpr int kZrThreads = 256;
-__global__ __launch_bounds__(kZrThreads) void cloud_zrange_kernel(Filters fs) {
-  const FilterDev& f = fs.arr ? fs.arr[blockIdx.y] : fs.one;
+__global__ __launch_bounds__(kZrThreads) void cloud_zrange_kernel(Filters fs, uint32_t base) {
+  const FilterDev& f = fs.arr ? fs.arr[base + blockIdx.y] : fs.one;
   if (!f.zr) return;  // uniform
   float r[9];
   rotation9(f.roll, f.pitch, 0.0f, r);
@@ -1114,8 +1114,9 @@ void launch_weight(const GridDev& g, const Filters& fs, const WorkItem* d_items,
     const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
     // the clouds' z ranges for the kernels' segment loads (a launch of a
     // wider index kind that allows them)
-    if (index_kind(g) != kIdxBuf && g.seg_bits && fs.count > 0 && (fs.arr || fs.one.n > 0))
-      cloud_zrange_kernel<<<dim3(kZrParts, fs.count), kZrThreads, 0, s>>>(fs);
+    if (index_kind(g) != kIdxBuf && g.seg_bits && (fs.arr || fs.one.n > 0))
+      for (uint32_t base = 0; base < fs.count; base += 65535u)
+        cloud_zrange_kernel<<<dim3(kZrParts, std::min(fs.count - base, 65535u)), kZrThreads, 0, s>>>(fs, base);
     if (!fs.arr) {
       if (fs.one.n == 0) return;
       if (ppw == 64) lanes_single_q<1>(g, fs.one, chunk, s);
@@ -1163,7 +1164,7 @@ void launch_weight_poses(const GridDev& g, const float* cx, const float* cy, con
   ppw = clamp_ppw(ppw);
   if (current_mapping() == 0) {
     const uint32_t chunk = (uint32_t)clamp_chunk(current_chunk());
-    if (index_kind(g) != kIdxBuf && g.seg_bits) cloud_zrange_kernel<<<dim3(kZrParts, 1), kZrThreads, 0, s>>>(single(f));
+    if (index_kind(g) != kIdxBuf && g.seg_bits) cloud_zrange_kernel<<<dim3(kZrParts, 1), kZrThreads, 0, s>>>(single(f), 0u);
     if (ppw == 64) lanes_poses_q<1>(g, f, out, chunk, s);
     else if (ppw == 32) lanes_poses_q<2>(g, f, out, chunk, s);
     else lanes_poses_q<4>(g, f, out, chunk, s);
